@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Throughput of the D2Q9 hot path on one MI355X.
+
+Workload: plain D2Q9 BGK at 4096x4096 in binary32, tau = 0.55, periodic in
+both axes, Guo body force g = (1e-6, 0), seeded non-overlapping disks
+(radius 4..24) on >= 5 % of the cells with bounce-back, starting from rest.
+One step = one fused collide-stream pass over the whole lattice.  Steps run
+as 100-step HIP-graph chunks after the tile shape has been autotuned.
+
+Prints one JSON line: MLUPS (all cells, solid included), the HBM roofline of
+the step kernel (72 algorithmic bytes per cell per step over the kernel's
+average duration from HIP events on its stream) and a CPU baseline: the
+binary32 restatement of the reference's WGSL (collide_stream + boundary
+passes, OpenMP over rows) timed on this host on a bounded sample of the same
+lattice without the body force, which the reference cannot express.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+  python bench.py --copy-calibration   # plain copy kernel, for byte counters
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "MLUPS + % HBM roofline, D2Q9 plain vs learned-collision at fixed vel L2 err"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_F32 = 72  # 9 populations read + 9 written, binary32
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(workload: str):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by the copy-kernel calibration), as committed in
+    profiles/<workload>_pmc.md, or None."""
+    p = ROOT / "profiles" / f"{workload}_pmc.md"
+    if not p.exists():
+        return None
+    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(scene, budget_s: float = 12.0):
+    """The reference's algorithm on the host cores: binary32 restatement of
+    collide_stream.wgsl + boundary.wgsl on the same 4096^2 disk lattice
+    (materials 1 / 4, no body force), timed over whole steps until about
+    budget_s seconds of work."""
+    from oracle import oracle as O
+    from simuverse_amd import workloads as W
+
+    ny, nx = scene.solid.shape
+    info = np.zeros((ny, nx), O.INFO_DTYPE)
+    info["material"] = np.where(scene.solid, 4, 1)
+    info["block_iter"] = -1
+    st = O.WgslState(info, 0.55)
+    st.init(fluid_ty=1)
+    st.step(0)  # first touch of the buffers
+    steps, t0, swap = 0, time.perf_counter(), 1
+    while True:
+        st.step(swap)
+        swap ^= 1
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": round(nx * ny * steps / el / 1e6, 2),
+        "unit": "MLUPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} steps of the 4096x4096 disk lattice (materials 1/4, no body force), "
+                  f"{el:.1f} s, binary32 WGSL restatement with OpenMP over rows",
+    }
+
+
+def run_copy_calibration(nbytes: int):
+    from simuverse_amd import lbm as L
+
+    ms = L.copy_bandwidth(nbytes, reps=20)
+    line = {"kernel": "k_copy16", "bytes_read": nbytes, "bytes_written": nbytes, "ms_per_copy": ms,
+            "GBps": 2 * nbytes / ms / 1e6}
+    print(json.dumps(line), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the lattice would shard by "
+              "y-slabs with a one-row halo exchange per step)", file=sys.stderr)
+        sys.exit(2)
+
+    from simuverse_amd import lbm as L
+    from simuverse_amd import workloads as W
+
+    if args.copy_calibration:
+        run_copy_calibration(args.copy_bytes)
+        return
+
+    n = args.n
+    tau, g = 0.55, 1e-6
+    t = time.perf_counter()
+    scene = W.periodic_disks(n, n)
+    log(f"scene {n}x{n}: {len(scene.disks)} disks, solid {scene.solid_fraction:.4f} ({time.perf_counter()-t:.1f} s)")
+    lat = L.Lattice(n, n, tau, mode=L.MODE_PHYSICS, force=(g, 0.0))
+    lat.write_materials(W.materials_from_solid(scene.solid))
+    lat.reset()
+    if args.variant >= 0:
+        lat.set_variant(args.variant)
+    elif not args.no_autotune:
+        v, ms = lat.autotune(steps=20)
+        log(f"autotune: variant {v} {L.variant_shapes()[v]} {ms*1e3:.1f} us/step")
+    variant = lat.variant()
+
+    lat.run(args.warmup)
+    lat.synchronize()
+    K = args.steps
+    wall0 = time.perf_counter()
+    ms = lat.run_timed(K)  # HIP events on the library's stream around K steps
+    lat.synchronize()
+    wall = time.perf_counter() - wall0
+    cells = n * n
+    sec = ms / 1e3
+    mlups = cells * K / sec / 1e6
+    step_us = ms * 1e3 / K
+    achieved = BYTES_PER_CELL_F32 * cells / (step_us * 1e-6) / 1e9
+    workload = f"d2q9_phys_f32_{n}x{n}"
+    r = lat.reduce()
+    log(f"{K} steps: {ms:.1f} ms events, {wall*1e3:.1f} ms wall, {step_us:.2f} us/step, {mlups:.0f} MLUPS, "
+        f"{achieved:.0f} GB/s; mass {r[0]:.9e} drag ({r[3]:.3e}, {r[4]:.3e})")
+    if not np.isfinite(r).all():
+        raise SystemExit("non-finite state after the timed run")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene)
+        log(f"cpu baseline: {cpu}")
+
+    traffic = traffic_from_profiles(workload)
+    line = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / K, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": workload,
+            "nx": n,
+            "ny": n,
+            "tau": tau,
+            "body_force": [g, 0.0],
+            "boundaries": "periodic x and y, half-way bounce-back on disks",
+            "solid_fraction": round(scene.solid_fraction, 5),
+            "disks": int(len(scene.disks)),
+            "seed": W.SEED,
+            "tile_variant": {"block": L.variant_shapes()[variant][0],
+                             "cells_per_thread": L.variant_shapes()[variant][1]},
+            "graph_chunk": 100,
+            "wall_ms": round(wall * 1e3, 3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,226 @@
+/*
+ * lbm.h -- C ABI of the MI355X (gfx950) D2Q9 lattice-Boltzmann hot path.
+ *
+ * This library replaces the wgpu compute pipelines, bind groups and buffers
+ * that simuverse's fluid node builds for its three LBM shaders.  Each entry
+ * point names the reference interface it stands in for (paths are inside the
+ * jinleili/simuverse tree):
+ *
+ *   D2Q9Node::new              simuverse/src/fluid/d2q9_node.rs:31-209
+ *   D2Q9Node::compute_by_pass  simuverse/src/fluid/d2q9_node.rs:302-312
+ *   D2Q9Node::reset / init     simuverse/src/fluid/d2q9_node.rs:211-213,
+ *                              assets/wgsl/lbm/init.wgsl:19-63
+ *   info_buf writes            simuverse/src/fluid/d2q9_node.rs:243-244,250-254,297-298
+ *   LbmUniform rewrite         simuverse/src/fluid/fluid_simulator.rs:175-193
+ *   macro_tex (rgba16float)    simuverse/src/fluid/d2q9_node.rs:91-104,
+ *                              assets/wgsl/lbm/collide_stream.wgsl:36,74
+ *
+ * Conventions
+ *   - Every function returns an int status: LBM_OK (0) or an LBM_ERR_* code.
+ *     Nothing aborts or throws across the ABI; lbm_last_error() returns the
+ *     message of the last failure on the calling thread.
+ *   - The caller owns host memory, the library owns device memory.
+ *   - Each handle owns one HIP stream; calls on one handle are stream
+ *     ordered.  A handle is not thread-safe; separate handles may be driven
+ *     from separate threads.
+ *   - Host-side population arrays are float32, structure-of-arrays, dense:
+ *     f[i * nx * ny + y * nx + x], i in 0..8 in the reference's direction
+ *     numbering (simuverse/src/fluid/mod.rs:22-52):
+ *         6 2 5        i : 0  1  2  3  4  5  6  7  8
+ *         3 0 1        ex: 0  1  0 -1  0  1 -1 -1  1
+ *         7 4 8        ey: 0  0 -1  0  1 -1 -1  1  1
+ *     (y grows downwards, as in the reference's lattice.)
+ */
+#ifndef SIMUVERSE_LBM_H
+#define SIMUVERSE_LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ABI_VERSION 1
+
+typedef struct lbm_handle lbm_handle;
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+    LBM_OK = 0,
+    LBM_ERR_INVALID = 1,     /* bad argument or handle state            */
+    LBM_ERR_HIP = 2,         /* a HIP runtime call failed               */
+    LBM_ERR_OOM = 3,         /* device allocation failed                */
+    LBM_ERR_UNSUPPORTED = 4, /* valid request this build does not do    */
+    LBM_ERR_NO_DEVICE = 5    /* no gfx950 device visible                */
+};
+
+/* ---- solver semantics ------------------------------------------------- */
+enum {
+    /* The reference shaders' arithmetic: truncated f32 weights
+     * (0.444444 / 0.111111 / 0.0277777), rho clamp [0.8, 1.2], post-collision
+     * clamp [0, max_i], accelerate cells (materials 3 and 6) with their
+     * block_iter countdown, periodic gather wrap in x and y, and the boundary
+     * pass's bounce-back with its outer-band skip
+     * (assets/wgsl/lbm/collide_stream.wgsl:25-88, boundary.wgsl:3-35). */
+    LBM_MODE_PARITY_WGSL = 0,
+    /* Textbook D2Q9 BGK: exact weights 4/9, 1/9, 1/36, no clamps, half-way
+     * bounce-back on solid cells, Guo body force, optional Zou-He velocity
+     * inlet at x = 0 with a zero-gradient outlet at x = nx-1. */
+    LBM_MODE_PHYSICS = 1
+};
+
+/* ---- population storage on the device --------------------------------- */
+enum {
+    LBM_STORAGE_F32 = 0,    /* f_i as binary32: 72 B per cell per step     */
+    LBM_STORAGE_F16_DEV = 1 /* (f_i - w_i) * 2^dev_shift as binary16 with
+                               binary32 arithmetic: 36 B per cell per step.
+                               Physics mode only. */
+};
+
+/* ---- x-edge treatment (physics mode) ----------------------------------- */
+enum {
+    LBM_X_PERIODIC = 0,
+    /* Zou-He velocity inlet (ux = inlet_ux, uy = 0) on the fluid cells of
+     * column x = 0; zero-gradient outlet on column x = nx-1 (its
+     * post-streaming populations are those of column nx-2). */
+    LBM_X_INLET_OUTLET = 1
+};
+/* y is periodic in both modes; channel walls are solid cells. */
+
+/* ---- materials: the reference's LatticeType (fluid/lattice.rs:15-24) --- */
+enum {
+    LBM_MAT_BULK = 1,
+    LBM_MAT_BOUNDARY = 2,       /* solid                                    */
+    LBM_MAT_INLET = 3,          /* accelerate cell in parity mode           */
+    LBM_MAT_OBSTACLE = 4,       /* solid                                    */
+    LBM_MAT_OUTLET = 5,
+    LBM_MAT_EXTERNAL_FORCE = 6, /* accelerate cell in parity mode           */
+    LBM_MAT_GHOST = 7
+};
+
+/* The reference's 16-byte LatticeInfo record (fluid/lattice.rs:5-11,
+ * assets/wgsl/lbm/struct/lattice_info.wgsl:2-8). */
+typedef struct lbm_lattice_info {
+    int32_t material;
+    int32_t block_iter;
+    float vx;
+    float vy;
+} lbm_lattice_info;
+
+/* Everything D2Q9Node::new derives from its settings
+ * (d2q9_node.rs:38-59: lattice size, tau, fluid_ty; LbmUniform). */
+typedef struct lbm_desc {
+    int32_t nx, ny;     /* lattice size; 2 <= nx, ny; nx*ny <= 2^31 / 9   */
+    float tau;          /* relaxation time; omega = 1/tau in binary32     */
+    int32_t mode;       /* LBM_MODE_*                                     */
+    int32_t storage;    /* LBM_STORAGE_*                                  */
+    int32_t fluid_ty;   /* parity-mode reset: 0 Poiseuille-biased start,
+                           1 plain start (LbmUniform.fluid_ty)            */
+    int32_t x_edge;     /* physics mode: LBM_X_*                          */
+    float inlet_ux;     /* physics mode, LBM_X_INLET_OUTLET: inlet ux     */
+    float force_x;      /* physics mode: uniform body force (Guo)         */
+    float force_y;
+    int32_t dev_shift;  /* LBM_STORAGE_F16_DEV: power-of-two scale of the
+                           stored deviations (0..14; 0 means the default 10) */
+    int32_t reserved[5];/* must be zero                                   */
+} lbm_desc;
+
+/* ---- library ----------------------------------------------------------- */
+int lbm_abi_version(void);
+const char* lbm_last_error(void);
+/* Number of visible HIP devices of architecture gfx950. */
+int lbm_device_count(int* count);
+
+/* ---- lifetime (D2Q9Node::new / drop) ----------------------------------- */
+/* Allocates both population buffers (ping-pong, like the two lattice_buf
+ * storage buffers at d2q9_node.rs:110-118), the material map (all
+ * LBM_MAT_BULK) and the macro field.  The populations are reset as by
+ * lbm_reset(). */
+int lbm_create(const lbm_desc* desc, lbm_handle** out);
+int lbm_destroy(lbm_handle* h);
+
+/* ---- inputs ------------------------------------------------------------ */
+/* queue.write_buffer(info_buf, 16 * offset, records) -- d2q9_node.rs:244,
+ * 250-254, 298.  Cells offset .. offset+count-1 in row-major order. */
+int lbm_write_lattice_info(lbm_handle* h, const lbm_lattice_info* info,
+                           size_t offset, size_t count);
+int lbm_read_lattice_info(lbm_handle* h, lbm_lattice_info* info,
+                          size_t offset, size_t count);
+/* Compact form: only the material of each cell (block_iter = -1, v = 0). */
+int lbm_write_materials(lbm_handle* h, const uint8_t* material, size_t offset,
+                        size_t count);
+
+/* init.wgsl:19-63 on both buffers: parity mode -- solids zeroed, f = w (or
+ * the Poiseuille-biased start when fluid_ty == 0) in buffer 0, 0 in buffer 1
+ * (the biased f1, f3 in both), expiring accelerate cells; physics mode --
+ * f = w_i (rest state at rho = 1) in buffer 0 and 1, solids zeroed.  The
+ * current buffer becomes 0. */
+int lbm_reset(lbm_handle* h);
+
+/* Seeded initial states, readback and checkpoints.  buf is 0 or 1; the
+ * array holds 9*nx*ny floats in the layout described at the top. */
+int lbm_write_populations(lbm_handle* h, int buf, const float* f);
+int lbm_read_populations(lbm_handle* h, int buf, float* f);
+
+/* LbmUniform rewrite with a new tau (fluid_simulator.rs:175-193). */
+int lbm_set_tau(lbm_handle* h, float tau);
+
+/* ---- stepping ---------------------------------------------------------- */
+/* One LBM step reading buffer `swap` and writing buffer 1-swap: the
+ * reference's collide_stream + boundary dispatch pair
+ * (d2q9_node.rs:302-312), fused into one kernel.  The current buffer
+ * becomes 1-swap.  When export_macro is non-zero the step also writes the
+ * macro field exactly as collide_stream.wgsl writes macro_tex: (ux, uy, rho)
+ * of the gathered pre-collision populations, rho after the clamp in parity
+ * mode, zero on solid cells. */
+int lbm_step(lbm_handle* h, int swap, int export_macro);
+/* nsteps steps starting from the current buffer, as 100-step HIP-graph
+ * chunks plus a remainder.  FluidSimulator::compute (fluid_simulator.rs:
+ * 217-232) is lbm_run(h, 2, 1).  export_macro applies to the last step. */
+int lbm_run(lbm_handle* h, int nsteps, int export_macro);
+int lbm_current_buffer(lbm_handle* h, int* buf);
+int lbm_synchronize(lbm_handle* h);
+/* lbm_run timed with HIP events recorded on the handle's stream; *ms is the
+ * elapsed device time of the nsteps steps. */
+int lbm_run_timed(lbm_handle* h, int nsteps, double* ms);
+
+/* ---- outputs ----------------------------------------------------------- */
+/* The macro field written by the last step that exported it (float32
+ * planes, nx*ny each; any pointer may be NULL). */
+int lbm_read_macro(lbm_handle* h, float* rho, float* ux, float* uy);
+/* The same field packed like macro_tex: rgba16float (ux, uy, rho, w) per
+ * texel, w = 1 on fluid and 0 on solid cells, round-to-nearest-even. */
+int lbm_read_macro_rgba16f(lbm_handle* h, uint16_t* rgba);
+/* Moments of the current state (the populations the next step gathers):
+ * the values the next step would export. */
+int lbm_macro(lbm_handle* h, float* rho, float* ux, float* uy);
+/* Domain-wide sums over fluid cells, accumulated in binary64 with a fixed
+ * reduction order: out[0] mass, out[1..2] momentum (sum of e_i f_i of the
+ * gathered populations), out[3..4] the force exerted on solid cells by the
+ * last step (momentum exchange over fluid-solid links). */
+int lbm_reduce(lbm_handle* h, double out[5]);
+
+/* ---- tuning and diagnostics (no reference counterpart: the reference
+ * dispatches a fixed 64x4 workgroup, d2q9_node.rs:45) --------------------- */
+/* Physics-mode step kernels are built for several tile shapes; every shape
+ * computes bit-identical results. */
+int lbm_variant_count(void);
+int lbm_variant_info(int variant, int* block, int* cells_per_thread);
+int lbm_set_variant(lbm_handle* h, int variant);
+int lbm_get_variant(lbm_handle* h, int* variant);
+/* Times every tile shape over `steps` steps on this handle's lattice and
+ * keeps the fastest.  The state is restored afterwards. */
+int lbm_autotune(lbm_handle* h, int steps, int* best_variant, double* ms_per_step);
+/* Steps per HIP-graph chunk used by lbm_run (even, default 100; 0 launches
+ * every step directly). */
+int lbm_set_graph_chunk(lbm_handle* h, int chunk);
+/* Device-to-device copy of `bytes` with a plain streaming kernel on the
+ * current device: the HBM ceiling a streaming kernel reaches, and the
+ * calibration run for byte counters. */
+int lbm_copy_bandwidth(size_t bytes, int reps, double* ms_per_copy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SIMUVERSE_LBM_H */

@@ -1,0 +1,210 @@
+"""numpy wrappers over the CPU oracle library (oracle/lib/liblbm_oracle.so).
+
+TEST INFRASTRUCTURE ONLY.  The tests, the smoke check and the CPU-baseline
+leg of the benchmark use this module as the checker; the product code in
+simuverse_amd/ never imports it.
+
+  WGSL restatement (binary32)   wgsl_restate.c, follows the reference shaders
+  physics oracle (binary64)     phys_oracle.c
+  MLP oracle (binary64)         mlp_oracle.c
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "lib" / "liblbm_oracle.so"
+
+INFO_DTYPE = np.dtype([("material", "<i4"), ("block_iter", "<i4"), ("vx", "<f4"), ("vy", "<f4")])
+
+_lib = None
+
+
+class PoParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("tau", C.c_double),
+        ("x_edge", C.c_int32),
+        ("inlet_ux", C.c_double),
+        ("gx", C.c_double),
+        ("gy", C.c_double),
+    ]
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        P = C.c_void_p
+        sig = {
+            "ws_init": (None, [C.c_int, C.c_int, C.c_int, P, P, P, P]),
+            "ws_collide_stream": (None, [C.c_int, C.c_int, C.c_float, P, P, P, P]),
+            "ws_boundary": (None, [C.c_int, C.c_int, P, P]),
+            "ws_compute_by_pass": (None, [C.c_int, C.c_int, C.c_float, P, P, P, P, C.c_int]),
+            "ws_run": (None, [C.c_int, C.c_int, C.c_float, P, P, P, P, C.c_int, C.c_int]),
+            "ws_weight": (C.c_float, [C.c_int]),
+            "ws_max_value": (C.c_float, [C.c_int]),
+            "ws_omega": (C.c_float, [C.c_float]),
+            "po_weight": (C.c_double, [C.c_int]),
+            "po_gather": (None, [C.POINTER(PoParams), P, P, C.c_int, C.c_int, P]),
+            "po_collide": (None, [C.POINTER(PoParams), P, P]),
+            "po_step": (None, [C.POINTER(PoParams), P, P, P]),
+            "po_run": (None, [C.POINTER(PoParams), P, P, P, C.c_int]),
+            "po_macro": (None, [C.POINTER(PoParams), P, P, P, P, P]),
+            "po_reduce": (None, [C.POINTER(PoParams), P, P, P]),
+            "po_restrict4": (None, [C.c_int, C.c_int, P, P]),
+            "mlp_param_count": (C.c_int, []),
+            "mlp_gelu": (C.c_double, [C.c_double]),
+            "mlp_gelu_grad": (C.c_double, [C.c_double]),
+            "mlp_forward": (None, [P, C.c_int, P, P]),
+            "mlp_loss_grad": (C.c_double, [P, C.c_int, P, P, P]),
+            "mlp_adam": (None, [P, P, P, P, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+# ---------------------------------------------------------------------------
+# WGSL restatement (parity mode)
+# ---------------------------------------------------------------------------
+
+
+class WgslState:
+    """The reference node's buffers: info_buf, two lattice buffers, macro_tex."""
+
+    def __init__(self, info: np.ndarray, tau: float):
+        self.info = np.ascontiguousarray(info, dtype=INFO_DTYPE).copy()
+        self.ny, self.nx = self.info.shape
+        self.tau = np.float32(tau)
+        self.buf = [np.zeros((9, self.ny, self.nx), np.float32) for _ in range(2)]
+        self.macro = np.zeros((self.ny, self.nx, 4), np.float32)
+
+    def init(self, fluid_ty: int = 1) -> None:
+        lib().ws_init(self.nx, self.ny, fluid_ty, _p(self.info), _p(self.buf[0]), _p(self.buf[1]), _p(self.macro))
+
+    def step(self, swap: int) -> None:
+        omega = lib().ws_omega(self.tau)
+        lib().ws_compute_by_pass(self.nx, self.ny, omega, _p(self.info), _p(self.buf[0]), _p(self.buf[1]),
+                                 _p(self.macro), swap)
+
+    def run(self, nsteps: int, start_swap: int = 0) -> int:
+        """nsteps alternating steps; returns the buffer holding the result."""
+        lib().ws_run(self.nx, self.ny, self.tau, _p(self.info), _p(self.buf[0]), _p(self.buf[1]), _p(self.macro),
+                     nsteps, start_swap)
+        return (start_swap + nsteps) & 1
+
+
+def wgsl_weights() -> np.ndarray:
+    return np.array([lib().ws_weight(i) for i in range(9)], np.float32)
+
+
+def wgsl_max_values() -> np.ndarray:
+    return np.array([lib().ws_max_value(i) for i in range(9)], np.float32)
+
+
+# ---------------------------------------------------------------------------
+# Physics oracle (binary64)
+# ---------------------------------------------------------------------------
+
+
+def phys_params(nx: int, ny: int, tau: float, x_edge: int = 0, inlet_ux: float = 0.0,
+                force: tuple[float, float] = (0.0, 0.0)) -> PoParams:
+    return PoParams(nx, ny, tau, x_edge, inlet_ux, force[0], force[1])
+
+
+class PhysState:
+    """Post-collision populations of the fluid cells, binary64."""
+
+    def __init__(self, params: PoParams, solid: np.ndarray, f: np.ndarray):
+        self.p = params
+        self.solid = np.ascontiguousarray(solid, dtype=np.uint8)
+        self.f = np.ascontiguousarray(f, dtype=np.float64).reshape(9, params.ny, params.nx).copy()
+        self.f[:, self.solid.astype(bool)] = 0.0
+        self._tmp = np.empty_like(self.f)
+
+    def run(self, nsteps: int) -> None:
+        lib().po_run(C.byref(self.p), _p(self.solid), _p(self.f), _p(self._tmp), nsteps)
+
+    def macro(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        shape = (self.p.ny, self.p.nx)
+        rho, ux, uy = (np.empty(shape) for _ in range(3))
+        lib().po_macro(C.byref(self.p), _p(self.solid), _p(self.f), _p(rho), _p(ux), _p(uy))
+        return rho, ux, uy
+
+    def reduce(self) -> np.ndarray:
+        out = np.zeros(5)
+        lib().po_reduce(C.byref(self.p), _p(self.solid), _p(self.f), _p(out))
+        return out
+
+    def gather(self, x: int, y: int) -> np.ndarray:
+        out = np.zeros(9)
+        lib().po_gather(C.byref(self.p), _p(self.solid), _p(self.f), x, y, _p(out))
+        return out
+
+
+def phys_collide(params: PoParams, f: np.ndarray) -> np.ndarray:
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(9)
+    lib().po_collide(C.byref(params), _p(f), _p(out))
+    return out
+
+
+def restrict4(field: np.ndarray) -> np.ndarray:
+    field = np.ascontiguousarray(field, dtype=np.float64)
+    ny, nx = field.shape
+    out = np.zeros((ny // 4, nx // 4))
+    lib().po_restrict4(nx, ny, _p(field), _p(out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# MLP oracle (binary64)
+# ---------------------------------------------------------------------------
+
+
+def mlp_param_count() -> int:
+    return lib().mlp_param_count()
+
+
+def mlp_forward(params: np.ndarray, x: np.ndarray) -> np.ndarray:
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 9)
+    y = np.zeros_like(x)
+    lib().mlp_forward(_p(params), x.shape[0], _p(x), _p(y))
+    return y
+
+
+def mlp_loss_grad(params: np.ndarray, x: np.ndarray, t: np.ndarray) -> tuple[float, np.ndarray]:
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 9)
+    t = np.ascontiguousarray(t, dtype=np.float64).reshape(-1, 9)
+    g = np.zeros_like(params)
+    loss = lib().mlp_loss_grad(_p(params), x.shape[0], _p(x), _p(t), _p(g))
+    return loss, g
+
+
+def mlp_adam(params: np.ndarray, grad: np.ndarray, m: np.ndarray, v: np.ndarray, step: int, lr: float,
+             beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> None:
+    for a in (params, grad, m, v):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    lib().mlp_adam(_p(params), _p(grad), _p(m), _p(v), step, lr, beta1, beta2, eps)

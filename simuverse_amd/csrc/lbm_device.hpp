@@ -1,0 +1,117 @@
+// lbm_device.hpp -- device-side building blocks of the gfx950 D2Q9 kernels.
+//
+// Direction numbering and constants follow simuverse/src/fluid/mod.rs:22-52
+// (the LbmUniform table the reference shaders read):
+//     6 2 5
+//     3 0 1
+//     7 4 8
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbm {
+
+__host__ __device__ constexpr int ex_of(int i)
+{
+    return (i == 1 || i == 5 || i == 8) ? 1 : (i == 3 || i == 6 || i == 7) ? -1 : 0;
+}
+__host__ __device__ constexpr int ey_of(int i)
+{
+    return (i == 4 || i == 7 || i == 8) ? 1 : (i == 2 || i == 5 || i == 6) ? -1 : 0;
+}
+__host__ __device__ constexpr int inv_of(int i)
+{
+    return i == 0 ? 0 : i == 1 ? 3 : i == 2 ? 4 : i == 3 ? 1 : i == 4 ? 2 : i == 5 ? 7 : i == 6 ? 8 : i == 7 ? 5 : 6;
+}
+
+// Physics-mode weights 4/9, 1/9, 1/36 rounded once to binary32.
+__host__ __device__ constexpr float w_phys(int i)
+{
+    return i == 0 ? 4.0f / 9.0f : (i <= 4 ? 1.0f / 9.0f : 1.0f / 36.0f);
+}
+// Parity-mode weights and post-collision caps: the reference's binary32
+// literals (mod.rs:40-48).  Their sum is 0.9999988, not 1.
+__host__ __device__ constexpr float w_wgsl(int i)
+{
+    return i == 0 ? 0.444444f : (i <= 4 ? 0.111111f : 0.0277777f);
+}
+__host__ __device__ constexpr float max_wgsl(int i)
+{
+    return i == 0 ? 0.6f : (i <= 4 ? 0.2222f : 0.1111f);
+}
+
+// Materials (fluid/lattice.rs:15-24, assets/wgsl/lbm/d2q9_fn.wgsl:19-24).
+__host__ __device__ inline bool mat_solid(uint32_t m) { return m == 2u || m == 4u; }
+__host__ __device__ inline bool mat_accel(uint32_t m) { return m == 3u || m == 6u; }
+
+// Population storage.  Kernels work on deviations g_i = f_i - w_i from the
+// rest state; the storage decides how a deviation is kept in memory.
+//   float     stores f_i itself (binary32)
+//   _Float16  stores (f_i - w_i) * 2^shift as binary16
+template <typename T>
+struct Store;
+
+template <>
+struct Store<float> {
+    static constexpr int bytes = 4;
+    __device__ static inline float load_dev(const float* p, uint32_t k, float w, float /*inv_scale*/)
+    {
+        return p[k] - w;
+    }
+    __device__ static inline void store_dev(float* p, uint32_t k, float g, float w, float /*scale*/)
+    {
+        p[k] = w + g;
+    }
+    __device__ static inline double load_f64(const float* p, uint32_t k, float /*w*/, float /*inv_scale*/)
+    {
+        return (double)p[k];
+    }
+};
+
+template <>
+struct Store<_Float16> {
+    static constexpr int bytes = 2;
+    __device__ static inline float load_dev(const _Float16* p, uint32_t k, float /*w*/, float inv_scale)
+    {
+        return (float)p[k] * inv_scale;
+    }
+    __device__ static inline void store_dev(_Float16* p, uint32_t k, float g, float /*w*/, float scale)
+    {
+        p[k] = (_Float16)(g * scale);
+    }
+    __device__ static inline double load_f64(const _Float16* p, uint32_t k, float w, float inv_scale)
+    {
+        return (double)w + (double)((float)p[k] * inv_scale);
+    }
+};
+
+// Arguments shared by the physics-mode kernels of one domain.
+struct PhysArgs {
+    int32_t nx, ny;
+    uint32_t plane;   // elements between direction planes (>= nx*ny)
+    float omega;      // 1/tau
+    float fcoef;      // Guo prefactor 1 - omega/2
+    float gx, gy;     // body force
+    float inlet_ux;   // Zou-He inlet velocity (x_edge == 1)
+    float scale;      // binary16 deviation scale 2^shift
+    float inv_scale;  // 2^-shift
+};
+
+// Arguments of the parity-mode (reference semantics) kernels.
+struct WgslArgs {
+    int32_t nx, ny;
+    uint32_t plane;
+    float omega;
+};
+
+// Linear block id -> logical tile id such that the blocks the hardware
+// dispatches to one XCD (id mod 8) get a contiguous range of tiles.
+__device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t total)
+{
+    const uint32_t q = total >> 3, r = total & 7u;
+    const uint32_t xcd = bid & 7u, local = bid >> 3;
+    return (xcd < r) ? xcd * (q + 1u) + local : r * (q + 1u) + (xcd - r) * q + local;
+}
+
+}  // namespace lbm

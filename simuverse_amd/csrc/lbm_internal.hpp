@@ -1,0 +1,46 @@
+// lbm_internal.hpp -- host-side launchers shared by the kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lbm_device.hpp"
+
+namespace lbm {
+
+struct StepVariant {
+    int block;           // threads per block
+    int cells_per_thread;
+};
+extern const StepVariant kStepVariants[];
+extern const int kNumStepVariants;
+
+hipError_t launch_phys_step(int variant, bool f16, int xedge, bool force, bool export_macro, const PhysArgs& a,
+                            const void* src, void* dst, const uint8_t* solid, float* macro, hipStream_t s);
+hipError_t launch_phys_macro(bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
+                             const uint8_t* solid, float* rho, float* ux, float* uy, hipStream_t s);
+hipError_t launch_phys_prime(bool f16, int xedge, const PhysArgs& a, void* buf, const uint8_t* solid,
+                             hipStream_t s);
+hipError_t launch_phys_fill(bool f16, const PhysArgs& a, void* buf, const uint8_t* solid, hipStream_t s);
+hipError_t launch_phys_reduce(bool f16, int xedge, const PhysArgs& a, const void* src, const uint8_t* solid,
+                              double* partial, int nblocks, double* out, hipStream_t s);
+
+hipError_t launch_wgsl_step(bool export_macro, const WgslArgs& a, const float* src, float* dst, uint8_t* mat,
+                            int32_t* biter, const float2* fvel, float* macro, hipStream_t s);
+hipError_t launch_wgsl_macro(const WgslArgs& a, const float* src, const uint8_t* mat, const float2* fvel,
+                             float* rho, float* ux, float* uy, hipStream_t s);
+hipError_t launch_wgsl_reset(const WgslArgs& a, int fluid_ty, float* A, float* B, uint8_t* mat, int32_t* biter,
+                             float2* fvel, float* macro, hipStream_t s);
+hipError_t launch_wgsl_reduce(const WgslArgs& a, const float* src, const uint8_t* mat, double* partial,
+                              int nblocks, double* out, hipStream_t s);
+
+hipError_t launch_f32_to_dev16(const float* in, void* out, uint32_t n, uint32_t plane, float scale, hipStream_t s);
+hipError_t launch_dev16_to_f32(const void* in, float* out, uint32_t n, uint32_t plane, float inv_scale,
+                               hipStream_t s);
+hipError_t launch_pack_rgba16f(const float* macro, uint32_t n, uint16_t* out, hipStream_t s);
+hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s);
+
+// Number of blocks of the binary64 reduction (partials = 5 doubles each).
+constexpr int kReduceBlocks = 1024;
+
+}  // namespace lbm

@@ -1,0 +1,813 @@
+// lbm_kernels.hip -- gfx950 kernels of the D2Q9 hot path (one domain).
+//
+// Layout in HBM: populations are structure-of-arrays, one plane per
+// direction, plane stride `plane` elements (nx*ny rounded up to 64), row
+// major inside a plane; two buffers ping-pong.  Materials are one byte per
+// cell in the reference's LatticeType numbering.
+//
+// Streaming is the pull form (a cell gathers f_i from x - e_i and writes
+// only its own slots), fused with the collision.  Bounce-back is folded in
+// by a push into the solid cell: a fluid cell x whose neighbour x + e_i is
+// solid also writes its post-collision f_i into slot inv(i) of that solid
+// cell.  The next step's gather then reads the bounced value at x - e_inv(i)
+// like any other population, so the gather has no branches and no extra
+// loads.  Each solid slot has exactly one writer (x = S + e_k for slot k of
+// solid S), so the fused step is race-free.  This is the reference's
+// boundary pass (assets/wgsl/lbm/boundary.wgsl:16-33: dst[S, inv(i)] :=
+// dst[S - e_i, i]) done by the producing cell instead of a second dispatch.
+#include "lbm_device.hpp"
+#include "lbm_internal.hpp"
+
+namespace lbm {
+
+// ---------------------------------------------------------------------------
+// Physics mode
+// ---------------------------------------------------------------------------
+
+// Post-streaming deviations g_i = f_i - w_i of the fluid cell (x, y):
+// pull with periodic y, periodic x or (x_edge 1) an own-slot bounce-back for
+// populations that would come from outside [0, nx); zero-gradient outlet
+// (column nx-1 gathers as column nx-2 when that cell is fluid) and Zou-He
+// inlet on column 0.
+template <typename T, int XEDGE>
+__device__ inline void phys_gather(const PhysArgs& a, const T* __restrict__ src,
+                                   const uint8_t* __restrict__ solid, int x, int y, float g[9])
+{
+    const int nx = a.nx, ny = a.ny;
+    int cx = x;
+    if (XEDGE == 1 && x == nx - 1 && !solid[(uint32_t)y * (uint32_t)nx + (uint32_t)(nx - 2)]) cx = nx - 2;
+    const int ym = (y == 0) ? ny - 1 : y - 1;
+    const int yp = (y == ny - 1) ? 0 : y + 1;
+    int xm = cx - 1, xp = cx + 1;
+    if (XEDGE == 0) {
+        if (xm < 0) xm = nx - 1;
+        if (xp >= nx) xp = 0;
+    }
+    const uint32_t rm = (uint32_t)ym * (uint32_t)nx, r0 = (uint32_t)y * (uint32_t)nx,
+                   rp = (uint32_t)yp * (uint32_t)nx;
+    const uint32_t cc = r0 + (uint32_t)cx;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int ex = ex_of(i), ey = ey_of(i);
+        const int sx = ex == 1 ? xm : (ex == -1 ? xp : cx);
+        const uint32_t srow = ey == 1 ? rm : (ey == -1 ? rp : r0);
+        bool oob = false;
+        if (XEDGE == 1 && ex != 0) oob = (sx < 0) || (sx >= nx);
+        if (oob)
+            g[i] = Store<T>::load_dev(src, cc + (uint32_t)inv_of(i) * a.plane, w_phys(i), a.inv_scale);
+        else
+            g[i] = Store<T>::load_dev(src, srow + (uint32_t)sx + (uint32_t)i * a.plane, w_phys(i),
+                                      a.inv_scale);
+    }
+    if (XEDGE == 1 && x == 0) {
+        // Zou & He velocity inlet, in deviations: w0+w2+w4+2(w3+w6+w7) = 1.
+        const float u0 = a.inlet_ux;
+        const float G = g[0] + g[2] + g[4] + 2.0f * (g[3] + g[6] + g[7]);
+        const float rho = (1.0f + G) / (1.0f - u0);
+        const float ru = rho * u0;
+        const float h = 0.5f * (g[4] - g[2]);
+        g[1] = g[3] + (2.0f / 3.0f) * ru;
+        g[5] = g[7] + h + (1.0f / 6.0f) * ru;
+        g[8] = g[6] - h + (1.0f / 6.0f) * ru;
+    }
+}
+
+template <bool FORCE>
+__device__ inline void phys_moments(const PhysArgs& a, const float g[9], float& rho, float& ux, float& uy)
+{
+    float dr = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dr += g[i];
+    const float jx = (g[1] - g[3]) + (g[5] - g[6]) + (g[8] - g[7]);
+    const float jy = (g[4] - g[2]) + (g[7] - g[5]) + (g[8] - g[6]);
+    rho = 1.0f + dr;
+    const float ir = 1.0f / rho;
+    if (FORCE) {
+        ux = (jx + 0.5f * a.gx) * ir;
+        uy = (jy + 0.5f * a.gy) * ir;
+    } else {
+        ux = jx * ir;
+        uy = jy * ir;
+    }
+}
+
+// BGK (+ Guo forcing) in deviation form: g*_i = g_i - omega (g_i - geq_i) + F_i
+// with geq_i = feq_i - w_i = w_i ((rho - 1) + rho (3 eu + 4.5 eu^2 - 1.5 u^2)).
+template <bool FORCE>
+__device__ inline void phys_collide(const PhysArgs& a, float g[9], float rho, float ux, float uy)
+{
+    const float dr = rho - 1.0f;
+    const float usq = 1.5f * (ux * ux + uy * uy);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float exf = (float)ex_of(i), eyf = (float)ey_of(i);
+        const float eu = exf * ux + eyf * uy;
+        const float geq = w_phys(i) * (dr + rho * (3.0f * eu + 4.5f * (eu * eu) - usq));
+        float out = g[i] - a.omega * (g[i] - geq);
+        if (FORCE) {
+            const float fx = 3.0f * (exf - ux) + 9.0f * eu * exf;
+            const float fy = 3.0f * (eyf - uy) + 9.0f * eu * eyf;
+            out += a.fcoef * w_phys(i) * (fx * a.gx + fy * a.gy);
+        }
+        g[i] = out;
+    }
+}
+
+template <typename T, int XEDGE, bool FORCE, bool EXPORT>
+__device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T* __restrict__ dst,
+                                 const uint8_t* __restrict__ solid, float* __restrict__ macro,
+                                 uint32_t n, int x, int y)
+{
+    const int nx = a.nx, ny = a.ny;
+    const uint32_t r0 = (uint32_t)y * (uint32_t)nx;
+    const uint32_t c = r0 + (uint32_t)x;
+    if (solid[c]) {
+        if (EXPORT) {
+            macro[c] = 0.0f;
+            macro[c + n] = 0.0f;
+            macro[c + 2 * n] = 0.0f;
+            macro[c + 3 * n] = 0.0f;
+        }
+        return;
+    }
+    float g[9];
+    phys_gather<T, XEDGE>(a, src, solid, x, y, g);
+    float rho, ux, uy;
+    phys_moments<FORCE>(a, g, rho, ux, uy);
+    if (EXPORT) {
+        macro[c] = ux;
+        macro[c + n] = uy;
+        macro[c + 2 * n] = rho;
+        macro[c + 3 * n] = 1.0f;
+    }
+    phys_collide<FORCE>(a, g, rho, ux, uy);
+
+    // Neighbour coordinates for the push (x + e_i).
+    const int ym = (y == 0) ? ny - 1 : y - 1;
+    const int yp = (y == ny - 1) ? 0 : y + 1;
+    int xm = x - 1, xp = x + 1;
+    if (XEDGE == 0) {
+        if (xm < 0) xm = nx - 1;
+        if (xp >= nx) xp = 0;
+    }
+    const uint32_t rm = (uint32_t)ym * (uint32_t)nx, rp = (uint32_t)yp * (uint32_t)nx;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        Store<T>::store_dev(dst, c + (uint32_t)i * a.plane, g[i], w_phys(i), a.scale);
+        if (i == 0) continue;
+        const int ex = ex_of(i), ey = ey_of(i);
+        const int tx = ex == 1 ? xp : (ex == -1 ? xm : x);
+        const uint32_t trow = ey == 1 ? rp : (ey == -1 ? rm : r0);
+        if (XEDGE == 1 && ex != 0 && (tx < 0 || tx >= nx)) continue;
+        const uint32_t t = trow + (uint32_t)tx;
+        if (solid[t]) Store<T>::store_dev(dst, t + (uint32_t)inv_of(i) * a.plane, g[i], w_phys(i), a.scale);
+    }
+}
+
+// One step of the whole lattice.  Block = BS threads on a row segment of
+// BS*CPT cells; thread t takes cells t, t+BS, ... so every load and store
+// instruction of a wavefront covers 64 consecutive cells of one plane.
+template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS, int CPT>
+__global__ __launch_bounds__(BS) void k_phys_step(PhysArgs a, const T* __restrict__ src,
+                                                  T* __restrict__ dst, const uint8_t* __restrict__ solid,
+                                                  float* __restrict__ macro, uint32_t nseg)
+{
+    const uint32_t total = nseg * (uint32_t)a.ny;
+    const uint32_t tile = xcd_remap(blockIdx.x, total);
+    const int y = (int)(tile / nseg);
+    const int x0 = (int)(tile % nseg) * (BS * CPT) + (int)threadIdx.x;
+    const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int x = x0 + k * BS;
+        if (x < a.nx) phys_cell<T, XEDGE, FORCE, EXPORT>(a, src, dst, solid, macro, n, x, y);
+    }
+}
+
+template <typename T, int XEDGE, bool FORCE>
+__global__ __launch_bounds__(256) void k_phys_macro(PhysArgs a, const T* __restrict__ src,
+                                                    const uint8_t* __restrict__ solid,
+                                                    float* __restrict__ rho_out, float* __restrict__ ux_out,
+                                                    float* __restrict__ uy_out)
+{
+    const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= n) return;
+    const int y = (int)(c / (uint32_t)a.nx), x = (int)(c % (uint32_t)a.nx);
+    if (solid[c]) {
+        rho_out[c] = ux_out[c] = uy_out[c] = 0.0f;
+        return;
+    }
+    float g[9], rho, ux, uy;
+    phys_gather<T, XEDGE>(a, src, solid, x, y, g);
+    phys_moments<FORCE>(a, g, rho, ux, uy);
+    rho_out[c] = rho;
+    ux_out[c] = ux;
+    uy_out[c] = uy;
+}
+
+// Copies every fluid cell's f_i into slot inv(i) of its solid neighbour
+// x + e_i, so that a state written from the host (or reset) carries its
+// bounce-back values where the next gather expects them.
+template <typename T, int XEDGE>
+__global__ __launch_bounds__(256) void k_phys_prime(PhysArgs a, T* __restrict__ buf,
+                                                    const uint8_t* __restrict__ solid)
+{
+    const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= n || solid[c]) return;
+    const int nx = a.nx, ny = a.ny;
+    const int y = (int)(c / (uint32_t)nx), x = (int)(c % (uint32_t)nx);
+#pragma unroll
+    for (int i = 1; i < 9; ++i) {
+        int tx = x + ex_of(i), ty = y + ey_of(i);
+        if (ty < 0) ty += ny;
+        else if (ty >= ny) ty -= ny;
+        if (XEDGE == 0) {
+            if (tx < 0) tx += nx;
+            else if (tx >= nx) tx -= nx;
+        } else if (tx < 0 || tx >= nx) {
+            continue;
+        }
+        const uint32_t t = (uint32_t)ty * (uint32_t)nx + (uint32_t)tx;
+        if (solid[t]) buf[t + (uint32_t)inv_of(i) * a.plane] = buf[c + (uint32_t)i * a.plane];
+    }
+}
+
+// Rest state: f = w on fluid cells (deviation 0), raw 0 on solid cells.
+template <typename T>
+__global__ __launch_bounds__(256) void k_phys_fill(PhysArgs a, T* __restrict__ buf,
+                                                   const uint8_t* __restrict__ solid)
+{
+    const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= n) return;
+    const bool s = solid[c] != 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        if (s) buf[c + (uint32_t)i * a.plane] = (T)0;
+        else Store<T>::store_dev(buf, c + (uint32_t)i * a.plane, 0.0f, w_phys(i), a.scale);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Parity mode: the reference shaders' arithmetic, fused.
+// ---------------------------------------------------------------------------
+
+template <bool EXPORT>
+__device__ inline void wgsl_cell(const WgslArgs& a, const float* __restrict__ src, float* __restrict__ dst,
+                                 uint8_t* __restrict__ mat, int32_t* __restrict__ biter,
+                                 const float2* __restrict__ fvel, float* __restrict__ macro, uint32_t n,
+                                 int x, int y)
+{
+    const int nx = a.nx, ny = a.ny;
+    const uint32_t c = (uint32_t)y * (uint32_t)nx + (uint32_t)x;
+    const uint32_t m = mat[c];
+    if (mat_solid(m)) {
+        if (EXPORT) {
+            macro[c] = 0.0f;
+            macro[c + n] = 0.0f;
+            macro[c + 2 * n] = 0.0f;
+            macro[c + 3 * n] = 0.0f;
+        }
+        return;
+    }
+    // streaming_in (layout_and_fn.wgsl:38-51): single wrap on both axes.
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+    float f[9];
+    float vx = 0.0f, vy = 0.0f, rho = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int ex = ex_of(i), ey = ey_of(i);
+        const int sx = ex == 1 ? xm : (ex == -1 ? xp : x);
+        const int sy = ey == 1 ? ym : (ey == -1 ? yp : y);
+        f[i] = src[(uint32_t)sy * (uint32_t)nx + (uint32_t)sx + (uint32_t)i * a.plane];
+        rho = rho + f[i];
+        vx = vx + (float)ex * f[i];
+        vy = vy + (float)ey * f[i];
+    }
+    rho = fminf(fmaxf(rho, 0.8f), 1.2f);
+    vx = vx / rho;
+    vy = vy / rho;
+    float F[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) F[i] = 0.0f;
+    if (mat_accel(m)) {
+        int32_t bi = biter[c];
+        if (bi > 0) {
+            bi = bi - 1;
+            if (bi == 0) mat[c] = 1;
+        }
+        biter[c] = bi;
+        const float2 fv = fvel[c];
+        vx = fv.x * 0.5f / rho;
+        vy = fv.y * 0.5f / rho;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) F[i] = w_wgsl(i) * 3.0f * ((float)ex_of(i) * fv.x + (float)ey_of(i) * fv.y);
+    }
+    if (EXPORT) {
+        macro[c] = vx;
+        macro[c + n] = vy;
+        macro[c + 2 * n] = rho;
+        macro[c + 3 * n] = 1.0f;
+    }
+    const float usqr = 1.5f * (vx * vx + vy * vy);
+    // boundary.wgsl:19 skips neighbours in the outer one-cell band.
+    const bool band = x <= 0 || y <= 0 || x >= nx - 1 || y >= ny - 1;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float eu = (float)ex_of(i) * vx + (float)ey_of(i) * vy;
+        const float feq = rho * w_wgsl(i) * (1.0f + 3.0f * eu + 4.5f * (eu * eu) - usqr);
+        float t = f[i] - a.omega * (f[i] - feq) + F[i];
+        if (t > max_wgsl(i)) t = max_wgsl(i);
+        else if (t < 0.0f) t = 0.0f;
+        bool pushed = false;
+        if (i != 0 && !band) {
+            const uint32_t tc = (uint32_t)(y + ey_of(i)) * (uint32_t)nx + (uint32_t)(x + ex_of(i));
+            if (mat_solid(mat[tc])) {
+                dst[tc + (uint32_t)inv_of(i) * a.plane] = t;
+                pushed = true;
+            }
+        }
+        dst[c + (uint32_t)i * a.plane] = pushed ? 0.0f : t;
+    }
+}
+
+template <bool EXPORT>
+__global__ __launch_bounds__(256) void k_wgsl_step(WgslArgs a, const float* __restrict__ src,
+                                                   float* __restrict__ dst, uint8_t* __restrict__ mat,
+                                                   int32_t* __restrict__ biter, const float2* __restrict__ fvel,
+                                                   float* __restrict__ macro, uint32_t nseg)
+{
+    const uint32_t total = nseg * (uint32_t)a.ny;
+    const uint32_t tile = xcd_remap(blockIdx.x, total);
+    const int y = (int)(tile / nseg);
+    const int x = (int)(tile % nseg) * 256 + (int)threadIdx.x;
+    if (x >= a.nx) return;
+    wgsl_cell<EXPORT>(a, src, dst, mat, biter, fvel, macro, (uint32_t)a.nx * (uint32_t)a.ny, x, y);
+}
+
+// Macro of the current state as the next collide_stream would export it,
+// without the accelerate-cell countdown.
+__global__ __launch_bounds__(256) void k_wgsl_macro(WgslArgs a, const float* __restrict__ src,
+                                                    const uint8_t* __restrict__ mat,
+                                                    const float2* __restrict__ fvel, float* __restrict__ rho_out,
+                                                    float* __restrict__ ux_out, float* __restrict__ uy_out)
+{
+    const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= n) return;
+    const int nx = a.nx, ny = a.ny;
+    const int y = (int)(c / (uint32_t)nx), x = (int)(c % (uint32_t)nx);
+    const uint32_t m = mat[c];
+    if (mat_solid(m)) {
+        rho_out[c] = ux_out[c] = uy_out[c] = 0.0f;
+        return;
+    }
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+    float vx = 0.0f, vy = 0.0f, rho = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int ex = ex_of(i), ey = ey_of(i);
+        const int sx = ex == 1 ? xm : (ex == -1 ? xp : x);
+        const int sy = ey == 1 ? ym : (ey == -1 ? yp : y);
+        const float f = src[(uint32_t)sy * (uint32_t)nx + (uint32_t)sx + (uint32_t)i * a.plane];
+        rho = rho + f;
+        vx = vx + (float)ex * f;
+        vy = vy + (float)ey * f;
+    }
+    rho = fminf(fmaxf(rho, 0.8f), 1.2f);
+    vx = vx / rho;
+    vy = vy / rho;
+    if (mat_accel(m)) {
+        const float2 fv = fvel[c];
+        vx = fv.x * 0.5f / rho;
+        vy = fv.y * 0.5f / rho;
+    }
+    rho_out[c] = rho;
+    ux_out[c] = vx;
+    uy_out[c] = vy;
+}
+
+// init.wgsl:19-63 over both buffers.
+__global__ __launch_bounds__(256) void k_wgsl_reset(WgslArgs a, int fluid_ty, float* __restrict__ A,
+                                                    float* __restrict__ B, uint8_t* __restrict__ mat,
+                                                    int32_t* __restrict__ biter, float2* __restrict__ fvel,
+                                                    float* __restrict__ macro)
+{
+    const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t m = mat[c];
+    if (mat_solid(m)) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            A[c + (uint32_t)i * a.plane] = 0.0f;
+            B[c + (uint32_t)i * a.plane] = 0.0f;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            A[c + (uint32_t)i * a.plane] = w_wgsl(i);
+            B[c + (uint32_t)i * a.plane] = 0.0f;
+        }
+        if (fluid_ty == 0) {
+            const float temp = w_wgsl(3) * 0.5f;
+            A[c + 1u * a.plane] = w_wgsl(1) + temp;
+            A[c + 3u * a.plane] = temp;
+            B[c + 1u * a.plane] = w_wgsl(1) + temp;
+            B[c + 3u * a.plane] = temp;
+        }
+    }
+    if (mat_accel(m) && biter[c] > 0) {
+        biter[c] = 0;
+        mat[c] = 1;
+        fvel[c] = make_float2(0.0f, 0.0f);
+    }
+    macro[c] = 0.0f;
+    macro[c + n] = 0.0f;
+    macro[c + 2 * n] = 0.0f;
+    macro[c + 3 * n] = 1.0f;
+}
+
+// ---------------------------------------------------------------------------
+// binary64 reductions: mass, momentum, momentum-exchange force.
+// ---------------------------------------------------------------------------
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block of 256 threads: 5 sums -> partial[blockIdx.x*5 + k], fixed order.
+__device__ inline void block_sum5(double v[5], double* __restrict__ partial)
+{
+    __shared__ double sh[4][5];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const double s = wave_sum(v[k]);
+        if (lane == 0) sh[wid][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        const int k = threadIdx.x;
+        partial[blockIdx.x * 5 + k] = ((sh[0][k] + sh[1][k]) + (sh[2][k] + sh[3][k]));
+    }
+}
+
+template <typename T, int XEDGE>
+__global__ __launch_bounds__(256) void k_phys_reduce(PhysArgs a, const T* __restrict__ src,
+                                                     const uint8_t* __restrict__ solid,
+                                                     double* __restrict__ partial)
+{
+    const int nx = a.nx, ny = a.ny;
+    const uint32_t n = (uint32_t)nx * (uint32_t)ny;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t c = blockIdx.x * 256u + threadIdx.x; c < n; c += gridDim.x * 256u) {
+        if (solid[c]) continue;
+        const int y = (int)(c / (uint32_t)nx), x = (int)(c % (uint32_t)nx);
+        int cx = x;
+        if (XEDGE == 1 && x == nx - 1 && !solid[c - 1]) cx = nx - 2;
+        const uint32_t cc = (uint32_t)y * (uint32_t)nx + (uint32_t)cx;
+        double f[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            int sx = cx - ex_of(i), sy = y - ey_of(i);
+            if (sy < 0) sy += ny;
+            else if (sy >= ny) sy -= ny;
+            bool oob = false;
+            if (XEDGE == 0) {
+                if (sx < 0) sx += nx;
+                else if (sx >= nx) sx -= nx;
+            } else {
+                oob = sx < 0 || sx >= nx;
+            }
+            if (oob) f[i] = Store<T>::load_f64(src, cc + (uint32_t)inv_of(i) * a.plane, w_phys(i), a.inv_scale);
+            else
+                f[i] = Store<T>::load_f64(src, (uint32_t)sy * (uint32_t)nx + (uint32_t)sx + (uint32_t)i * a.plane,
+                                          w_phys(i), a.inv_scale);
+        }
+        if (XEDGE == 1 && x == 0) {
+            const double u0 = (double)a.inlet_ux;
+            const double rho = (f[0] + f[2] + f[4] + 2.0 * (f[3] + f[6] + f[7])) / (1.0 - u0);
+            f[1] = f[3] + (2.0 / 3.0) * rho * u0;
+            f[5] = f[7] + 0.5 * (f[4] - f[2]) + (1.0 / 6.0) * rho * u0;
+            f[8] = f[6] - 0.5 * (f[4] - f[2]) + (1.0 / 6.0) * rho * u0;
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            acc[0] += f[i];
+            acc[1] += (double)ex_of(i) * f[i];
+            acc[2] += (double)ey_of(i) * f[i];
+        }
+#pragma unroll
+        for (int i = 1; i < 9; ++i) {
+            int tx = x + ex_of(i), ty = y + ey_of(i);
+            if (ty < 0) ty += ny;
+            else if (ty >= ny) ty -= ny;
+            if (XEDGE == 0) {
+                if (tx < 0) tx += nx;
+                else if (tx >= nx) tx -= nx;
+            } else if (tx < 0 || tx >= nx) {
+                continue;
+            }
+            const uint32_t t = (uint32_t)ty * (uint32_t)nx + (uint32_t)tx;
+            if (!solid[t]) continue;
+            const double v = Store<T>::load_f64(src, t + (uint32_t)inv_of(i) * a.plane, w_phys(i), a.inv_scale);
+            acc[3] += 2.0 * (double)ex_of(i) * v;
+            acc[4] += 2.0 * (double)ey_of(i) * v;
+        }
+    }
+    block_sum5(acc, partial);
+}
+
+__global__ __launch_bounds__(256) void k_wgsl_reduce(WgslArgs a, const float* __restrict__ src,
+                                                     const uint8_t* __restrict__ mat,
+                                                     double* __restrict__ partial)
+{
+    const int nx = a.nx, ny = a.ny;
+    const uint32_t n = (uint32_t)nx * (uint32_t)ny;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t c = blockIdx.x * 256u + threadIdx.x; c < n; c += gridDim.x * 256u) {
+        if (mat_solid(mat[c])) continue;
+        const int y = (int)(c / (uint32_t)nx), x = (int)(c % (uint32_t)nx);
+        const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+        const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const int ex = ex_of(i), ey = ey_of(i);
+            const int sx = ex == 1 ? xm : (ex == -1 ? xp : x);
+            const int sy = ey == 1 ? ym : (ey == -1 ? yp : y);
+            const double f = (double)src[(uint32_t)sy * (uint32_t)nx + (uint32_t)sx + (uint32_t)i * a.plane];
+            acc[0] += f;
+            acc[1] += (double)ex * f;
+            acc[2] += (double)ey * f;
+        }
+        const bool band = x <= 0 || y <= 0 || x >= nx - 1 || y >= ny - 1;
+        if (band) continue;
+#pragma unroll
+        for (int i = 1; i < 9; ++i) {
+            const uint32_t t = (uint32_t)(y + ey_of(i)) * (uint32_t)nx + (uint32_t)(x + ex_of(i));
+            if (!mat_solid(mat[t])) continue;
+            const double v = (double)src[t + (uint32_t)inv_of(i) * a.plane];
+            acc[3] += 2.0 * (double)ex_of(i) * v;
+            acc[4] += 2.0 * (double)ey_of(i) * v;
+        }
+    }
+    block_sum5(acc, partial);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__ partial, int nblocks,
+                                                      double* __restrict__ out)
+{
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += 256) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc[k] += partial[b * 5 + k];
+    }
+    __shared__ double sh[4][5];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const double s = wave_sum(acc[k]);
+        if (lane == 0) sh[wid][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        const int k = threadIdx.x;
+        out[k] = (sh[0][k] + sh[1][k]) + (sh[2][k] + sh[3][k]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Format conversion and export.
+// ---------------------------------------------------------------------------
+
+// Dense host layout f[i*n + c] (binary32) <-> device planes of deviations.
+__global__ __launch_bounds__(256) void k_f32_to_dev16(const float* __restrict__ in, _Float16* __restrict__ out,
+                                                      uint32_t n, uint32_t plane, float scale)
+{
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= n) return;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+        out[c + (uint32_t)i * plane] = (_Float16)((in[c + (uint32_t)i * n] - w_phys(i)) * scale);
+}
+
+__global__ __launch_bounds__(256) void k_dev16_to_f32(const _Float16* __restrict__ in, float* __restrict__ out,
+                                                      uint32_t n, uint32_t plane, float inv_scale)
+{
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= n) return;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+        out[c + (uint32_t)i * n] = w_phys(i) + (float)in[c + (uint32_t)i * plane] * inv_scale;
+}
+
+// (ux, uy, rho, w) planes -> rgba16float texels, round to nearest even.
+__global__ __launch_bounds__(256) void k_pack_rgba16f(const float* __restrict__ macro, uint32_t n,
+                                                      uint16_t* __restrict__ out)
+{
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= n) return;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const _Float16 h = (_Float16)macro[c + (uint32_t)k * n];
+        out[4 * c + k] = __builtin_bit_cast(uint16_t, h);
+    }
+}
+
+// Plain copy used to calibrate HBM byte counters and as the bandwidth
+// ceiling a streaming kernel can reach.
+__global__ __launch_bounds__(256) void k_copy16(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4)
+{
+    for (size_t k = blockIdx.x * 256ull + threadIdx.x; k < n4; k += (size_t)gridDim.x * 256ull) dst[k] = src[k];
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+
+static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS, int CPT>
+static hipError_t launch_phys_variant(const PhysArgs& a, const void* src, void* dst, const uint8_t* solid,
+                                      float* macro, hipStream_t s)
+{
+    const uint32_t nseg = (uint32_t)((a.nx + BS * CPT - 1) / (BS * CPT));
+    const uint32_t total = nseg * (uint32_t)a.ny;
+    hipLaunchKernelGGL((k_phys_step<T, XEDGE, FORCE, EXPORT, BS, CPT>), dim3(total), dim3(BS), 0, s, a,
+                       (const T*)src, (T*)dst, solid, macro, nseg);
+    return hipGetLastError();
+}
+
+// Tile shapes the step kernel is built for: (block size, cells per thread).
+const StepVariant kStepVariants[] = {
+    {256, 1}, {256, 2}, {256, 4}, {256, 8}, {128, 4}, {512, 2}, {512, 4}, {1024, 4},
+};
+const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
+
+template <typename T, int XEDGE, bool FORCE, bool EXPORT>
+static hipError_t dispatch_variant(int v, const PhysArgs& a, const void* src, void* dst, const uint8_t* solid,
+                                   float* macro, hipStream_t s)
+{
+    switch (v) {
+    case 0: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 1>(a, src, dst, solid, macro, s);
+    case 1: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 2>(a, src, dst, solid, macro, s);
+    case 2: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 4>(a, src, dst, solid, macro, s);
+    case 3: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 8>(a, src, dst, solid, macro, s);
+    case 4: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 128, 4>(a, src, dst, solid, macro, s);
+    case 5: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 512, 2>(a, src, dst, solid, macro, s);
+    case 6: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 512, 4>(a, src, dst, solid, macro, s);
+    case 7: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 1024, 4>(a, src, dst, solid, macro, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T, int XEDGE>
+static hipError_t dispatch_force(int v, bool force, bool exp, const PhysArgs& a, const void* src, void* dst,
+                                 const uint8_t* solid, float* macro, hipStream_t s)
+{
+    if (force)
+        return exp ? dispatch_variant<T, XEDGE, true, true>(v, a, src, dst, solid, macro, s)
+                   : dispatch_variant<T, XEDGE, true, false>(v, a, src, dst, solid, macro, s);
+    return exp ? dispatch_variant<T, XEDGE, false, true>(v, a, src, dst, solid, macro, s)
+               : dispatch_variant<T, XEDGE, false, false>(v, a, src, dst, solid, macro, s);
+}
+
+hipError_t launch_phys_step(int variant, bool f16, int xedge, bool force, bool export_macro, const PhysArgs& a,
+                            const void* src, void* dst, const uint8_t* solid, float* macro, hipStream_t s)
+{
+    if (f16)
+        return xedge ? dispatch_force<_Float16, 1>(variant, force, export_macro, a, src, dst, solid, macro, s)
+                     : dispatch_force<_Float16, 0>(variant, force, export_macro, a, src, dst, solid, macro, s);
+    return xedge ? dispatch_force<float, 1>(variant, force, export_macro, a, src, dst, solid, macro, s)
+                 : dispatch_force<float, 0>(variant, force, export_macro, a, src, dst, solid, macro, s);
+}
+
+hipError_t launch_phys_macro(bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
+                             const uint8_t* solid, float* rho, float* ux, float* uy, hipStream_t s)
+{
+    const unsigned nb = blocks_for((size_t)a.nx * a.ny, 256);
+#define LBM_MACRO(T, XE, FO) \
+    hipLaunchKernelGGL((k_phys_macro<T, XE, FO>), dim3(nb), dim3(256), 0, s, a, (const T*)src, solid, rho, ux, uy)
+    if (f16) {
+        if (xedge) { if (force) LBM_MACRO(_Float16, 1, true); else LBM_MACRO(_Float16, 1, false); }
+        else { if (force) LBM_MACRO(_Float16, 0, true); else LBM_MACRO(_Float16, 0, false); }
+    } else {
+        if (xedge) { if (force) LBM_MACRO(float, 1, true); else LBM_MACRO(float, 1, false); }
+        else { if (force) LBM_MACRO(float, 0, true); else LBM_MACRO(float, 0, false); }
+    }
+#undef LBM_MACRO
+    return hipGetLastError();
+}
+
+hipError_t launch_phys_prime(bool f16, int xedge, const PhysArgs& a, void* buf, const uint8_t* solid,
+                             hipStream_t s)
+{
+    const unsigned nb = blocks_for((size_t)a.nx * a.ny, 256);
+    if (f16) {
+        if (xedge) hipLaunchKernelGGL((k_phys_prime<_Float16, 1>), dim3(nb), dim3(256), 0, s, a, (_Float16*)buf, solid);
+        else hipLaunchKernelGGL((k_phys_prime<_Float16, 0>), dim3(nb), dim3(256), 0, s, a, (_Float16*)buf, solid);
+    } else {
+        if (xedge) hipLaunchKernelGGL((k_phys_prime<float, 1>), dim3(nb), dim3(256), 0, s, a, (float*)buf, solid);
+        else hipLaunchKernelGGL((k_phys_prime<float, 0>), dim3(nb), dim3(256), 0, s, a, (float*)buf, solid);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_phys_fill(bool f16, const PhysArgs& a, void* buf, const uint8_t* solid, hipStream_t s)
+{
+    const unsigned nb = blocks_for((size_t)a.nx * a.ny, 256);
+    if (f16) hipLaunchKernelGGL((k_phys_fill<_Float16>), dim3(nb), dim3(256), 0, s, a, (_Float16*)buf, solid);
+    else hipLaunchKernelGGL((k_phys_fill<float>), dim3(nb), dim3(256), 0, s, a, (float*)buf, solid);
+    return hipGetLastError();
+}
+
+hipError_t launch_phys_reduce(bool f16, int xedge, const PhysArgs& a, const void* src, const uint8_t* solid,
+                              double* partial, int nblocks, double* out, hipStream_t s)
+{
+    if (f16) {
+        if (xedge) hipLaunchKernelGGL((k_phys_reduce<_Float16, 1>), dim3(nblocks), dim3(256), 0, s, a, (const _Float16*)src, solid, partial);
+        else hipLaunchKernelGGL((k_phys_reduce<_Float16, 0>), dim3(nblocks), dim3(256), 0, s, a, (const _Float16*)src, solid, partial);
+    } else {
+        if (xedge) hipLaunchKernelGGL((k_phys_reduce<float, 1>), dim3(nblocks), dim3(256), 0, s, a, (const float*)src, solid, partial);
+        else hipLaunchKernelGGL((k_phys_reduce<float, 0>), dim3(nblocks), dim3(256), 0, s, a, (const float*)src, solid, partial);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, partial, nblocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_wgsl_step(bool export_macro, const WgslArgs& a, const float* src, float* dst, uint8_t* mat,
+                            int32_t* biter, const float2* fvel, float* macro, hipStream_t s)
+{
+    const uint32_t nseg = (uint32_t)((a.nx + 255) / 256);
+    const uint32_t total = nseg * (uint32_t)a.ny;
+    if (export_macro)
+        hipLaunchKernelGGL((k_wgsl_step<true>), dim3(total), dim3(256), 0, s, a, src, dst, mat, biter, fvel, macro, nseg);
+    else
+        hipLaunchKernelGGL((k_wgsl_step<false>), dim3(total), dim3(256), 0, s, a, src, dst, mat, biter, fvel, macro, nseg);
+    return hipGetLastError();
+}
+
+hipError_t launch_wgsl_macro(const WgslArgs& a, const float* src, const uint8_t* mat, const float2* fvel,
+                             float* rho, float* ux, float* uy, hipStream_t s)
+{
+    const unsigned nb = blocks_for((size_t)a.nx * a.ny, 256);
+    hipLaunchKernelGGL(k_wgsl_macro, dim3(nb), dim3(256), 0, s, a, src, mat, fvel, rho, ux, uy);
+    return hipGetLastError();
+}
+
+hipError_t launch_wgsl_reset(const WgslArgs& a, int fluid_ty, float* A, float* B, uint8_t* mat, int32_t* biter,
+                             float2* fvel, float* macro, hipStream_t s)
+{
+    const unsigned nb = blocks_for((size_t)a.nx * a.ny, 256);
+    hipLaunchKernelGGL(k_wgsl_reset, dim3(nb), dim3(256), 0, s, a, fluid_ty, A, B, mat, biter, fvel, macro);
+    return hipGetLastError();
+}
+
+hipError_t launch_wgsl_reduce(const WgslArgs& a, const float* src, const uint8_t* mat, double* partial,
+                              int nblocks, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_wgsl_reduce, dim3(nblocks), dim3(256), 0, s, a, src, mat, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, partial, nblocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_f32_to_dev16(const float* in, void* out, uint32_t n, uint32_t plane, float scale, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_f32_to_dev16, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, (_Float16*)out, n, plane, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_dev16_to_f32(const void* in, float* out, uint32_t n, uint32_t plane, float inv_scale,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dev16_to_f32, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const _Float16*)in, out, n,
+                       plane, inv_scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_rgba16f(const float* macro, uint32_t n, uint16_t* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack_rgba16f, dim3(blocks_for(n, 256)), dim3(256), 0, s, macro, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s)
+{
+    const size_t n4 = bytes / 16;
+    hipLaunchKernelGGL(k_copy16, dim3(8192), dim3(256), 0, s, (const float4*)src, (float4*)dst, n4);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

@@ -1,0 +1,287 @@
+"""ctypes binding of ``include/lbm.h`` (the gfx950 D2Q9 library).
+
+The binding mirrors the reference's LBM node (simuverse/src/fluid/
+d2q9_node.rs:13-312): ``Lattice`` owns what ``D2Q9Node`` owns, ``step`` is
+``compute_by_pass``, ``reset`` is the init shader, ``write_lattice_info`` is
+the ``info_buf`` upload.  There is no CPU fallback: loading raises when the
+HIP library has not been built, and every call raises ``LbmError`` with the
+library's message when it returns a non-zero status.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libsimuverse_lbm.so"
+
+# status codes
+LBM_OK = 0
+# modes, storage, x edges, materials (include/lbm.h)
+MODE_PARITY_WGSL = 0
+MODE_PHYSICS = 1
+STORAGE_F32 = 0
+STORAGE_F16_DEV = 1
+X_PERIODIC = 0
+X_INLET_OUTLET = 1
+MAT_BULK, MAT_BOUNDARY, MAT_INLET, MAT_OBSTACLE, MAT_OUTLET, MAT_EXTERNAL_FORCE, MAT_GHOST = 1, 2, 3, 4, 5, 6, 7
+
+LATTICE_INFO_DTYPE = np.dtype([("material", "<i4"), ("block_iter", "<i4"), ("vx", "<f4"), ("vy", "<f4")])
+
+
+class LbmError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"lbm error {code}: {message}")
+        self.code = code
+
+
+class LbmDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("tau", C.c_float),
+        ("mode", C.c_int32),
+        ("storage", C.c_int32),
+        ("fluid_ty", C.c_int32),
+        ("x_edge", C.c_int32),
+        ("inlet_ux", C.c_float),
+        ("force_x", C.c_float),
+        ("force_y", C.c_float),
+        ("dev_shift", C.c_int32),
+        ("reserved", C.c_int32 * 5),
+    ]
+
+
+class LatticeInfoRec(C.Structure):
+    _fields_ = [("material", C.c_int32), ("block_iter", C.c_int32), ("vx", C.c_float), ("vy", C.c_float)]
+
+
+# (name, restype, argtypes) of every entry point include/lbm.h declares.
+_P = C.c_void_p
+_SIGNATURES = [
+    ("lbm_abi_version", C.c_int, []),
+    ("lbm_last_error", C.c_char_p, []),
+    ("lbm_device_count", C.c_int, [C.POINTER(C.c_int)]),
+    ("lbm_create", C.c_int, [C.POINTER(LbmDesc), C.POINTER(_P)]),
+    ("lbm_destroy", C.c_int, [_P]),
+    ("lbm_write_lattice_info", C.c_int, [_P, _P, C.c_size_t, C.c_size_t]),
+    ("lbm_read_lattice_info", C.c_int, [_P, _P, C.c_size_t, C.c_size_t]),
+    ("lbm_write_materials", C.c_int, [_P, _P, C.c_size_t, C.c_size_t]),
+    ("lbm_reset", C.c_int, [_P]),
+    ("lbm_write_populations", C.c_int, [_P, C.c_int, _P]),
+    ("lbm_read_populations", C.c_int, [_P, C.c_int, _P]),
+    ("lbm_set_tau", C.c_int, [_P, C.c_float]),
+    ("lbm_step", C.c_int, [_P, C.c_int, C.c_int]),
+    ("lbm_run", C.c_int, [_P, C.c_int, C.c_int]),
+    ("lbm_current_buffer", C.c_int, [_P, C.POINTER(C.c_int)]),
+    ("lbm_synchronize", C.c_int, [_P]),
+    ("lbm_run_timed", C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
+    ("lbm_read_macro", C.c_int, [_P, _P, _P, _P]),
+    ("lbm_read_macro_rgba16f", C.c_int, [_P, _P]),
+    ("lbm_macro", C.c_int, [_P, _P, _P, _P]),
+    ("lbm_reduce", C.c_int, [_P, _P]),
+    ("lbm_variant_count", C.c_int, []),
+    ("lbm_variant_info", C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("lbm_set_variant", C.c_int, [_P, C.c_int]),
+    ("lbm_get_variant", C.c_int, [_P, C.POINTER(C.c_int)]),
+    ("lbm_autotune", C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    ("lbm_set_graph_chunk", C.c_int, [_P, C.c_int]),
+    ("lbm_copy_bandwidth", C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+]
+
+_lib = None
+
+
+def library_path() -> Path:
+    return _LIB_PATH
+
+
+def load_library() -> C.CDLL:
+    """Loads the HIP library once; raises when it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        raise ImportError(
+            f"{_LIB_PATH} is missing: build the gfx950 library first (python -m simuverse_amd.build); "
+            "there is no CPU fallback"
+        )
+    lib = C.CDLL(str(_LIB_PATH))
+    for name, res, args in _SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != LBM_OK:
+        raise LbmError(rc, load_library().lbm_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(load_library().lbm_device_count(C.byref(n)))
+    return n.value
+
+
+def variant_shapes() -> list[tuple[int, int]]:
+    lib = load_library()
+    out = []
+    for v in range(lib.lbm_variant_count()):
+        b, c = C.c_int(), C.c_int()
+        _check(lib.lbm_variant_info(v, C.byref(b), C.byref(c)))
+        out.append((b.value, c.value))
+    return out
+
+
+def copy_bandwidth(nbytes: int, reps: int = 20) -> float:
+    """Milliseconds per device copy of nbytes with the plain streaming kernel."""
+    ms = C.c_double()
+    _check(load_library().lbm_copy_bandwidth(nbytes, reps, C.byref(ms)))
+    return ms.value
+
+
+class Lattice:
+    """One D2Q9 domain on the current HIP device."""
+
+    def __init__(
+        self,
+        nx: int,
+        ny: int,
+        tau: float,
+        mode: int = MODE_PHYSICS,
+        storage: int = STORAGE_F32,
+        fluid_ty: int = 1,
+        x_edge: int = X_PERIODIC,
+        inlet_ux: float = 0.0,
+        force: tuple[float, float] = (0.0, 0.0),
+        dev_shift: int = 0,
+    ):
+        self._lib = load_library()
+        self.nx, self.ny = int(nx), int(ny)
+        self.n = self.nx * self.ny
+        self.mode = mode
+        d = LbmDesc()
+        d.nx, d.ny, d.tau = self.nx, self.ny, tau
+        d.mode, d.storage, d.fluid_ty, d.x_edge = mode, storage, fluid_ty, x_edge
+        d.inlet_ux, d.force_x, d.force_y, d.dev_shift = inlet_ux, force[0], force[1], dev_shift
+        self.desc = d
+        h = C.c_void_p()
+        _check(self._lib.lbm_create(C.byref(d), C.byref(h)))
+        self._h = h
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _check(self._lib.lbm_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- inputs -----------------------------------------------------------
+    def write_lattice_info(self, info: np.ndarray, offset: int = 0) -> None:
+        info = np.ascontiguousarray(info, dtype=LATTICE_INFO_DTYPE).reshape(-1)
+        _check(self._lib.lbm_write_lattice_info(self._h, _ptr(info), offset, info.size))
+
+    def read_lattice_info(self) -> np.ndarray:
+        out = np.zeros(self.n, dtype=LATTICE_INFO_DTYPE)
+        _check(self._lib.lbm_read_lattice_info(self._h, _ptr(out), 0, self.n))
+        return out.reshape(self.ny, self.nx)
+
+    def write_materials(self, material: np.ndarray, offset: int = 0) -> None:
+        m = np.ascontiguousarray(material, dtype=np.uint8).reshape(-1)
+        _check(self._lib.lbm_write_materials(self._h, _ptr(m), offset, m.size))
+
+    def reset(self) -> None:
+        _check(self._lib.lbm_reset(self._h))
+
+    def write_populations(self, f: np.ndarray, buf: int | None = None) -> None:
+        f = np.ascontiguousarray(f, dtype=np.float32).reshape(9, self.ny, self.nx)
+        _check(self._lib.lbm_write_populations(self._h, self.current_buffer() if buf is None else buf, _ptr(f)))
+
+    def read_populations(self, buf: int | None = None) -> np.ndarray:
+        out = np.empty((9, self.ny, self.nx), dtype=np.float32)
+        _check(self._lib.lbm_read_populations(self._h, self.current_buffer() if buf is None else buf, _ptr(out)))
+        return out
+
+    def set_tau(self, tau: float) -> None:
+        _check(self._lib.lbm_set_tau(self._h, tau))
+
+    # -- stepping ---------------------------------------------------------
+    def step(self, swap: int, export_macro: bool = True) -> None:
+        _check(self._lib.lbm_step(self._h, swap, int(export_macro)))
+
+    def run(self, nsteps: int, export_macro: bool = False) -> None:
+        _check(self._lib.lbm_run(self._h, nsteps, int(export_macro)))
+
+    def current_buffer(self) -> int:
+        b = C.c_int()
+        _check(self._lib.lbm_current_buffer(self._h, C.byref(b)))
+        return b.value
+
+    def synchronize(self) -> None:
+        _check(self._lib.lbm_synchronize(self._h))
+
+    def run_timed(self, nsteps: int) -> float:
+        ms = C.c_double()
+        _check(self._lib.lbm_run_timed(self._h, nsteps, C.byref(ms)))
+        return ms.value
+
+    def set_variant(self, v: int) -> None:
+        _check(self._lib.lbm_set_variant(self._h, v))
+
+    def variant(self) -> int:
+        v = C.c_int()
+        _check(self._lib.lbm_get_variant(self._h, C.byref(v)))
+        return v.value
+
+    def autotune(self, steps: int = 20) -> tuple[int, float]:
+        v, ms = C.c_int(), C.c_double()
+        _check(self._lib.lbm_autotune(self._h, steps, C.byref(v), C.byref(ms)))
+        return v.value, ms.value
+
+    def set_graph_chunk(self, chunk: int) -> None:
+        _check(self._lib.lbm_set_graph_chunk(self._h, chunk))
+
+    # -- outputs ----------------------------------------------------------
+    def read_macro(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        rho, ux, uy = (np.empty((self.ny, self.nx), np.float32) for _ in range(3))
+        _check(self._lib.lbm_read_macro(self._h, _ptr(rho), _ptr(ux), _ptr(uy)))
+        return rho, ux, uy
+
+    def read_macro_rgba16f(self) -> np.ndarray:
+        out = np.empty((self.ny, self.nx, 4), np.uint16)
+        _check(self._lib.lbm_read_macro_rgba16f(self._h, _ptr(out)))
+        return out.view(np.float16)
+
+    def macro(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        rho, ux, uy = (np.empty((self.ny, self.nx), np.float32) for _ in range(3))
+        _check(self._lib.lbm_macro(self._h, _ptr(rho), _ptr(ux), _ptr(uy)))
+        return rho, ux, uy
+
+    def reduce(self) -> np.ndarray:
+        out = np.zeros(5, np.float64)
+        _check(self._lib.lbm_reduce(self._h, _ptr(out)))
+        return out
+
+
+def exported_symbols() -> list[str]:
+    return [name for name, _, _ in _SIGNATURES]
+

@@ -1,0 +1,67 @@
+"""The C-ABI library loads without a GPU, exports every entry point that
+include/lbm.h declares, and rejects bad descriptors before touching a
+device."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from simuverse_amd import lbm as L
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADERS = sorted((ROOT / "include").glob("*.h"))
+
+
+def declared_functions():
+    names = []
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(lbm_\w+)\s*\(", text, flags=re.M)
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.load_library()
+    names = declared_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in include/ but not exported"
+
+
+def test_binding_covers_every_declared_symbol():
+    assert set(declared_functions()) <= set(L.exported_symbols()) | {"lbm_batch_placeholder"}
+
+
+def test_abi_version_and_struct_layout():
+    lib = L.load_library()
+    assert lib.lbm_abi_version() == 1
+    assert C.sizeof(L.LbmDesc) == 64
+    assert C.sizeof(L.LatticeInfoRec) == 16  # LatticeInfo, fluid/lattice.rs:5-11
+
+
+@pytest.mark.parametrize(
+    "kw, needle",
+    [
+        (dict(nx=1, ny=8), "both sides"),
+        (dict(tau=0.0), "tau"),
+        (dict(mode=7), "mode"),
+        (dict(mode=L.MODE_PARITY_WGSL, storage=L.STORAGE_F16_DEV), "binary32"),
+        (dict(mode=L.MODE_PARITY_WGSL, force=(1e-6, 0.0)), "parity"),
+        (dict(x_edge=L.X_INLET_OUTLET, nx=2), "nx >= 3"),
+        (dict(dev_shift=20), "dev_shift"),
+    ],
+)
+def test_invalid_descriptors_are_rejected(kw, needle):
+    args = dict(nx=8, ny=8, tau=0.6)
+    args.update(kw)
+    with pytest.raises(L.LbmError) as ei:
+        L.Lattice(**args)
+    assert needle in str(ei.value)
+    assert ei.value.code in (1, 4)
+
+
+def test_null_handle_is_an_error_not_a_crash():
+    lib = L.load_library()
+    assert lib.lbm_step(None, 0, 0) == 1
+    assert b"null handle" in lib.lbm_last_error()

@@ -1,0 +1,322 @@
+"""Parity of the gfx950 library (through its C ABI) with the oracle.
+
+Parity mode is compared with the binary32 restatement of the reference's
+WGSL bit for bit; physics mode with the binary64 oracle under the
+tolerances written in each test.  Sizes are the ones the oracle finishes in
+seconds; the full 4096^2 lattice is checked through properties that do not
+depend on size (mass conservation, the momentum identity).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from simuverse_amd import lbm as L
+from simuverse_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gfx950():
+    n = L.device_count()
+    assert n >= 1, "no gfx950 device visible"
+
+
+def visible_mask(solid: np.ndarray) -> np.ndarray:
+    """(9, ny, nx) mask of the population slots a fluid cell can ever read:
+    every slot of a fluid cell, and slot k of a solid cell S when the cell
+    S + e_k (periodic) is fluid (it pulls direction k from S)."""
+    ny, nx = solid.shape
+    vis = np.zeros((9, ny, nx), bool)
+    fluid = ~solid
+    for k in range(9):
+        pulls_from_me = np.roll(np.roll(fluid, -W.EX[k], axis=1), -W.EY[k], axis=0)
+        vis[k] = fluid | (solid & pulls_from_me)
+    return vis
+
+
+def rel_l2(a, b, mask=None):
+    if mask is not None:
+        a, b = a[mask], b[mask]
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+# ---------------------------------------------------------------------------
+# Parity mode: reference WGSL semantics, bit for bit
+# ---------------------------------------------------------------------------
+
+
+def run_parity_pair(info, f0, f1, tau, steps, frames_api=False):
+    st = O.WgslState(info, tau)
+    st.buf[0][:] = f0
+    st.buf[1][:] = f1
+    cur = st.run(steps)
+    ny, nx = info.shape
+    lat = L.Lattice(nx, ny, tau, mode=L.MODE_PARITY_WGSL, fluid_ty=1)
+    lat.write_lattice_info(info)
+    lat.write_populations(f0, 0)
+    lat.write_populations(f1, 1)
+    if frames_api:
+        for _ in range(steps // 2):  # FluidSimulator::compute: step(0), step(1)
+            lat.step(0)
+            lat.step(1)
+    else:
+        lat.run(steps, export_macro=True)
+    assert lat.current_buffer() == cur
+    return st, lat, cur
+
+
+def test_parity_c1_cylinder_1000_steps_bit_exact():
+    """256x256 channel past a cylinder r=16 at (64,128), tau=0.6, inlet
+    accelerate column vx=0.05, f0 = w + U(-1e-4, 1e-4), 1000 steps."""
+    print("case: parity C1 256x256 tau=0.6 1000 steps", flush=True)
+    info = W.cylinder_parity_info()
+    f0, f1 = W.parity_initial_state(info)
+    st, lat, cur = run_parity_pair(info, f0, f1, 0.6, 1000)
+    solid = np.isin(info["material"], (2, 4))
+    vis = visible_mask(solid)
+    for b in (cur, cur ^ 1):
+        g = lat.read_populations(b)
+        mism = np.count_nonzero(g[vis] != st.buf[b][vis])
+        assert mism == 0, f"buffer {b}: {mism} visible slots differ, rel L2 {rel_l2(g, st.buf[b], vis):.3e}"
+    rho, ux, uy = lat.read_macro()
+    np.testing.assert_array_equal(rho, st.macro[..., 2])
+    np.testing.assert_array_equal(ux, st.macro[..., 0])
+    np.testing.assert_array_equal(uy, st.macro[..., 1])
+    # the flow is developed: a non-trivial wake behind the cylinder
+    assert np.max(np.abs(uy)) > 1e-3
+    # the macro of the current state equals what the next step exports
+    r2, u2, v2 = lat.macro()
+    st.step(cur)
+    np.testing.assert_array_equal(r2, st.macro[..., 2])
+    np.testing.assert_array_equal(u2, st.macro[..., 0])
+    np.testing.assert_array_equal(v2, st.macro[..., 1])
+
+
+def test_parity_reference_scene_with_reset_and_edits():
+    """The reference's own default scene (init_lattice_material, Poiseuille,
+    600x375, tau = 0.56) from init.wgsl's Poiseuille-biased start, driven
+    frame by frame like FluidSimulator::compute, with an obstacle click and
+    an external-force drag written between frames (d2q9_node.rs:215-300)."""
+    print("case: parity reference scene 600x375 tau=0.56", flush=True)
+    nx, ny = 600, 375
+    tau = float(np.float32(3.0) * np.float32(0.02) + np.float32(0.5))  # d2q9_node.rs:50
+    info = W.reference_poiseuille_info(nx, ny)
+    st = O.WgslState(info, tau)
+    st.init(fluid_ty=0)
+    lat = L.Lattice(nx, ny, tau, mode=L.MODE_PARITY_WGSL, fluid_ty=0)
+    lat.write_lattice_info(info)
+    lat.reset()
+    np.testing.assert_array_equal(lat.read_populations(0), st.buf[0])
+    np.testing.assert_array_equal(lat.read_populations(1), st.buf[1])
+
+    def frame():
+        st.step(0)
+        st.step(1)
+        lat.step(0)
+        lat.step(1)
+
+    for _ in range(40):
+        frame()
+    # add_obstacle at (300, 180): rows min_y .. min_y + 2r of the info buffer
+    cx, cy, r = 300, 180, 28
+    rows = slice(cy - r, cy + r)
+    yy, xx = np.mgrid[cy - r:cy + r, 0:nx]
+    disk = (xx + 0.5 - (cx + 0.5)) ** 2 + (yy + 0.5 - (cy + 0.5)) ** 2 <= r * r
+    block = st.info[rows].copy()
+    block["material"][disk] = 4
+    block["block_iter"][disk] = -1
+    block["vx"][disk] = 0.0
+    block["vy"][disk] = 0.0
+    st.info[rows] = block
+    lat.write_lattice_info(block, offset=(cy - r) * nx)
+    # add_external_force: material 6, block_iter 90 records along a stroke
+    # over fluid cells (a stroke over a solid cell turns it into fluid and
+    # exposes slots the reference's boundary pass writes racily)
+    for k in range(20):
+        x, y = 300 + 3 * k, 60 + k
+        rec = np.array([(6, 90, 0.08, 0.04)], dtype=L.LATTICE_INFO_DTYPE)
+        st.info[y, x] = rec[0]
+        lat.write_lattice_info(rec, offset=y * nx + x)
+    for _ in range(60):
+        frame()
+    solid = np.isin(st.info["material"], (2, 4))
+    vis = visible_mask(solid)
+    for b in (0, 1):
+        g = lat.read_populations(b)
+        mism = np.count_nonzero(g[vis] != st.buf[b][vis])
+        assert mism == 0, f"buffer {b}: {mism} visible slots differ"
+    got = lat.read_lattice_info()
+    np.testing.assert_array_equal(got["material"], st.info["material"])
+    np.testing.assert_array_equal(got["block_iter"], st.info["block_iter"])
+    rho, ux, uy = lat.read_macro()
+    np.testing.assert_array_equal(rho, st.macro[..., 2])
+    np.testing.assert_array_equal(ux, st.macro[..., 0])
+    rgba = lat.read_macro_rgba16f()
+    np.testing.assert_array_equal(rgba, st.macro.astype(np.float16))
+
+
+# ---------------------------------------------------------------------------
+# Physics mode against the binary64 oracle
+# ---------------------------------------------------------------------------
+
+
+def physics_pair(nx, ny, tau, solid, f0, steps, x_edge=0, inlet_ux=0.0, force=(0.0, 0.0),
+                 storage=L.STORAGE_F32):
+    p = O.phys_params(nx, ny, tau, x_edge=x_edge, inlet_ux=inlet_ux, force=force)
+    st = O.PhysState(p, solid, f0)
+    st.run(steps)
+    lat = L.Lattice(nx, ny, tau, mode=L.MODE_PHYSICS, storage=storage, x_edge=x_edge, inlet_ux=inlet_ux,
+                    force=force)
+    lat.write_materials(W.materials_from_solid(solid))
+    lat.write_populations(f0.astype(np.float32), 0)
+    lat.run(steps)
+    return st, lat
+
+
+def macro_errors(st, lat, solid):
+    r0, u0, v0 = st.macro()
+    r1, u1, v1 = lat.macro()
+    fl = ~solid
+    du = np.hypot(u1 - u0, v1 - v0)[fl]
+    umag = np.hypot(u0, v0)[fl]
+    return {
+        "rho_l2": rel_l2(r1[fl].astype(np.float64), r0[fl]),
+        "rho_max": float(np.max(np.abs(r1[fl] - r0[fl]))),
+        "u_l2": float(np.linalg.norm(du) / np.linalg.norm(umag)),
+        "u_max": float(np.max(du)),
+        "u_scale": float(np.max(umag)),
+    }
+
+
+def test_physics_c1_channel_cylinder_vs_fp64():
+    """256x256 channel: Zou-He inlet ux=0.05, zero-gradient outlet, walls,
+    cylinder r=16 at (64,128), tau=0.6, f0 = w + U(-1e-4,1e-4), 1000 steps.
+    binary32 storage and arithmetic against binary64: rho and u relative L2
+    <= 1e-5."""
+    print("case: physics C1 256x256 1000 steps", flush=True)
+    nx = ny = 256
+    solid = W.cylinder_channel_solid()
+    f0 = W.rest_state(nx, ny, solid, amplitude=1e-4)
+    st, lat = physics_pair(nx, ny, 0.6, solid, f0, 1000, x_edge=1, inlet_ux=0.05)
+    e = macro_errors(st, lat, solid)
+    print("physics C1 errors vs fp64:", e, flush=True)
+    assert e["rho_l2"] < 1e-5 and e["u_l2"] < 1e-5
+    assert e["u_max"] < 1e-5 * 10
+
+
+def test_physics_periodic_disks_force_vs_fp64():
+    """The roofline scene at 256^2: periodic, Guo force g=1e-6 (x), seeded
+    disks r in 4..24 on >= 5 % of the cells, tau = 0.55, 1000 steps."""
+    print("case: physics periodic disks 256x256 1000 steps", flush=True)
+    nx = ny = 256
+    scene = W.periodic_disks(nx, ny)
+    f0 = W.rest_state(nx, ny, scene.solid)
+    st, lat = physics_pair(nx, ny, 0.55, scene.solid, f0, 1000, force=(1e-6, 0.0))
+    e = macro_errors(st, lat, scene.solid)
+    print("physics periodic-disk errors vs fp64:", e, flush=True)
+    assert e["rho_l2"] < 1e-6 and e["u_l2"] < 1e-4
+    r_gpu = lat.reduce()
+    r_ref = st.reduce()
+    np.testing.assert_allclose(r_gpu[0], r_ref[0], rtol=1e-6)
+    np.testing.assert_allclose(r_gpu[1:], r_ref[1:], rtol=1e-3, atol=1e-9)
+
+
+def test_physics_reduce_identities_small():
+    """binary64 reductions on the GPU obey the same exact identities as the
+    oracle: P(t+1) - P(t) = g N_fluid - drag(t), mass conserved."""
+    print("case: physics reduce identities 96x80", flush=True)
+    nx, ny = 96, 80
+    scene = W.periodic_disks(nx, ny, 0.08, 3, 6, seed=9)
+    g = (2e-5, -1e-5)
+    lat = L.Lattice(nx, ny, 0.55, force=g)
+    lat.write_materials(W.materials_from_solid(scene.solid))
+    lat.write_populations(W.rest_state(nx, ny, scene.solid, amplitude=1e-3).astype(np.float32), 0)
+    nfl = int((~scene.solid).sum())
+    m0 = lat.reduce()[0]
+    for _ in range(4):
+        r0 = lat.reduce()
+        lat.run(1)
+        r1 = lat.reduce()
+        # binary32 populations: the identity holds to the rounding of f
+        assert abs((r1[1] - r0[1]) - (g[0] * nfl - r1[3])) < 1e-5 * nfl * 1e-3
+        assert abs((r1[2] - r0[2]) - (g[1] * nfl - r1[4])) < 1e-5 * nfl * 1e-3
+    assert abs(lat.reduce()[0] - m0) < 1e-6 * m0
+
+
+def test_tile_variants_and_graphs_bit_identical():
+    print("case: variants + graphs 300x200", flush=True)
+    nx, ny = 300, 200
+    scene = W.periodic_disks(nx, ny, 0.05, 4, 10, seed=4)
+    f0 = W.rest_state(nx, ny, scene.solid, amplitude=1e-3).astype(np.float32)
+    outs = []
+    for v in range(len(L.variant_shapes())):
+        lat = L.Lattice(nx, ny, 0.58, force=(1e-5, 0.0))
+        lat.write_materials(W.materials_from_solid(scene.solid))
+        lat.write_populations(f0, 0)
+        lat.set_variant(v)
+        lat.set_graph_chunk(0 if v % 2 else 100)
+        lat.run(202)
+        outs.append(lat.read_populations())
+        lat.close()
+    for k, o in enumerate(outs[1:], 1):
+        assert np.array_equal(o, outs[0]), f"variant {k} differs"
+
+
+def test_checkpoint_resume_bit_identical():
+    print("case: checkpoint/resume 128x96", flush=True)
+    nx, ny = 128, 96
+    solid = W.cylinder_channel_solid(nx, ny, cx=40, cy=48, r=8)
+    f0 = W.rest_state(nx, ny, solid, amplitude=1e-4).astype(np.float32)
+    a = L.Lattice(nx, ny, 0.6, x_edge=L.X_INLET_OUTLET, inlet_ux=0.05)
+    a.write_materials(W.materials_from_solid(solid))
+    a.write_populations(f0, 0)
+    a.run(150)
+    ref = a.read_populations()
+    b = L.Lattice(nx, ny, 0.6, x_edge=L.X_INLET_OUTLET, inlet_ux=0.05)
+    b.write_materials(W.materials_from_solid(solid))
+    b.write_populations(f0, 0)
+    b.run(77)
+    ck = b.read_populations()
+    c = L.Lattice(nx, ny, 0.6, x_edge=L.X_INLET_OUTLET, inlet_ux=0.05)
+    c.write_materials(W.materials_from_solid(solid))
+    c.write_populations(ck, 0)
+    c.run(73)
+    fl = ~solid
+    assert np.array_equal(c.read_populations()[:, fl], ref[:, fl])
+
+
+def test_f16_deviation_storage_vs_fp64():
+    """binary16 storage of (f - w) 2^10 with binary32 arithmetic, periodic
+    disks + force, 1000 steps: rho relative L2 <= 1e-5, u relative L2 <= 5e-2
+    (the flow speed is ~1e-4, so binary16's 2^-11 relative step on the
+    deviations is visible)."""
+    print("case: f16 storage periodic disks 256x256 1000 steps", flush=True)
+    nx = ny = 256
+    scene = W.periodic_disks(nx, ny)
+    f0 = W.rest_state(nx, ny, scene.solid)
+    st, lat = physics_pair(nx, ny, 0.55, scene.solid, f0, 1000, force=(1e-6, 0.0), storage=L.STORAGE_F16_DEV)
+    e = macro_errors(st, lat, scene.solid)
+    print("f16 errors vs fp64:", e, flush=True)
+    assert e["rho_l2"] < 1e-5 and e["u_l2"] < 5e-2
+
+
+def test_full_size_4096_mass_and_momentum_properties():
+    """The 4096^2 roofline lattice (periodic, disks >= 5 %, g = 1e-6,
+    tau = 0.55): the binary64 reduction shows mass conserved and the
+    momentum identity, 100 steps."""
+    print("case: 4096x4096 properties 100 steps", flush=True)
+    n = 4096
+    scene = W.periodic_disks(n, n)
+    g = 1e-6
+    lat = L.Lattice(n, n, 0.55, force=(g, 0.0))
+    lat.write_materials(W.materials_from_solid(scene.solid))
+    lat.reset()
+    nfl = int((~scene.solid).sum())
+    r0 = lat.reduce()
+    lat.run(100)
+    r1 = lat.reduce()
+    assert abs(r1[0] - r0[0]) < 1e-6 * r0[0]
+    lat.run(1)
+    r2 = lat.reduce()
+    assert abs((r2[1] - r1[1]) - (g * nfl - r2[3])) < 1e-3 * g * nfl
