@@ -106,6 +106,7 @@ def main():
     ap.add_argument("--variant", type=int, default=-1)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--graph-chunk", type=int, default=100, help="steps per HIP graph (0: direct launches)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -128,6 +129,7 @@ def main():
     lat = L.Lattice(n, n, tau, mode=L.MODE_PHYSICS, force=(g, 0.0))
     lat.write_materials(W.materials_from_solid(scene.solid))
     lat.reset()
+    lat.set_graph_chunk(args.graph_chunk)
     if args.variant >= 0:
         lat.set_variant(args.variant)
     elif not args.no_autotune:
@@ -185,7 +187,7 @@ def main():
             "seed": W.SEED,
             "tile_variant": {"block": L.variant_shapes()[variant][0],
                              "cells_per_thread": L.variant_shapes()[variant][1]},
-            "graph_chunk": 100,
+            "graph_chunk": args.graph_chunk,
             "wall_ms": round(wall * 1e3, 3),
         },
         "roofline": {

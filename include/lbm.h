@@ -72,11 +72,15 @@ enum {
 
 /* ---- population storage on the device --------------------------------- */
 enum {
-    LBM_STORAGE_F32 = 0,    /* f_i as binary32: 72 B per cell per step     */
+    LBM_STORAGE_F32 = 0,    /* binary32, 72 B per cell per step: f_i in
+                               parity mode (the reference's buffers), the
+                               deviation f_i - w_i in physics mode         */
     LBM_STORAGE_F16_DEV = 1 /* (f_i - w_i) * 2^dev_shift as binary16 with
                                binary32 arithmetic: 36 B per cell per step.
                                Physics mode only. */
 };
+/* lbm_write_populations / lbm_read_populations always exchange f_i itself
+ * in binary32; the library converts to and from its storage. */
 
 /* ---- x-edge treatment (physics mode) ----------------------------------- */
 enum {
@@ -162,6 +166,15 @@ int lbm_reset(lbm_handle* h);
  * array holds 9*nx*ny floats in the layout described at the top. */
 int lbm_write_populations(lbm_handle* h, int buf, const float* f);
 int lbm_read_populations(lbm_handle* h, int buf, float* f);
+
+/* Checkpoint / resume (no reference counterpart: the reference only
+ * re-runs init, fluid_simulator.rs:210-215).  The state is the device
+ * image of the current population buffer, lbm_state_size() bytes; loading
+ * it into a handle with the same descriptor and lattice info and stepping
+ * reproduces an uninterrupted run bit for bit. */
+int lbm_state_size(lbm_handle* h, size_t* bytes);
+int lbm_save_state(lbm_handle* h, void* out, size_t bytes);
+int lbm_load_state(lbm_handle* h, const void* in, size_t bytes);
 
 /* LbmUniform rewrite with a new tau (fluid_simulator.rs:175-193). */
 int lbm_set_tau(lbm_handle* h, float tau);

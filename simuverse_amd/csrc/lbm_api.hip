@@ -60,6 +60,8 @@ struct lbm_handle {
     int32_t* biter = nullptr;
     float2* fvel = nullptr;
     float* macro = nullptr;  // planes ux, uy, rho, w (stride n)
+    uint32_t* bits = nullptr;  // packed solid mask (physics mode)
+    uint32_t words = 0;        // mask words per row
     float* tmp3 = nullptr;   // lbm_macro output planes
     double* partial = nullptr;
     double* red = nullptr;
@@ -124,7 +126,7 @@ hipError_t enqueue_step(lbm_handle* h, int swap, bool export_macro)
     void* dst = h->buf[swap ^ 1];
     if (h->physics())
         return launch_phys_step(h->variant, h->f16, h->d.x_edge, h->force(), export_macro, h->phys(), src, dst,
-                                h->mat, h->macro, h->stream);
+                                h->bits, h->words, h->macro, h->stream);
     return launch_wgsl_step(export_macro, h->wgsl(), (const float*)src, (float*)dst, h->mat, h->biter, h->fvel,
                             h->macro, h->stream);
 }
@@ -179,6 +181,16 @@ int reprime(lbm_handle* h, int b)
 {
     if (!h->physics()) return LBM_OK;
     HIP_TRY(launch_phys_prime(h->f16, h->d.x_edge, h->phys(), h->buf[b], h->mat, h->stream));
+    return LBM_OK;
+}
+
+// Physics mode: the packed solid mask the step kernel reads, rebuilt from
+// the material bytes after every material write.
+int rebuild_mask(lbm_handle* h)
+{
+    if (!h->physics()) return LBM_OK;
+    HIP_TRY(launch_build_solid_bits(h->mat, h->d.nx, h->d.ny, h->d.x_edge == LBM_X_PERIODIC ? 1 : 0, h->words,
+                                    h->bits, h->stream));
     return LBM_OK;
 }
 
@@ -240,7 +252,7 @@ void free_handle(lbm_handle* h)
 {
     drop_graphs(h);
     for (void* p : {h->buf[0], h->buf[1], (void*)h->mat, (void*)h->biter, (void*)h->fvel, (void*)h->macro,
-                    (void*)h->tmp3, (void*)h->partial, (void*)h->red})
+                    (void*)h->tmp3, (void*)h->partial, (void*)h->red, (void*)h->bits})
         if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -253,7 +265,9 @@ int upload_info(lbm_handle* h, const uint8_t* m, const int32_t* bi, const float2
     HIP_TRY(hipMemcpyAsync(h->mat + offset, m, count, hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->biter + offset, bi, count * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->fvel + offset, v, count * sizeof(float2), hipMemcpyHostToDevice, h->stream));
-    int rc = reprime(h, h->cur);
+    int rc = rebuild_mask(h);
+    if (rc != LBM_OK) return rc;
+    rc = reprime(h, h->cur);
     if (rc != LBM_OK) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     return LBM_OK;
@@ -297,6 +311,7 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
     h->plane = (h->n + 63u) & ~63u;
     h->f16 = desc->storage == LBM_STORAGE_F16_DEV;
     h->elem = h->f16 ? 2 : 4;
+    h->words = solid_mask_words(desc->nx);
     h->omega = 1.0f / desc->tau;
     const int shift = desc->dev_shift == 0 ? 10 : desc->dev_shift;
     h->scale = std::ldexp(1.0f, shift);
@@ -315,7 +330,8 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
         (rc = alloc((void**)&h->mat, n)) || (rc = alloc((void**)&h->biter, n * sizeof(int32_t))) ||
         (rc = alloc((void**)&h->fvel, n * sizeof(float2))) || (rc = alloc((void**)&h->macro, 4 * n * sizeof(float))) ||
         (rc = alloc((void**)&h->partial, (size_t)kReduceBlocks * 5 * sizeof(double))) ||
-        (rc = alloc((void**)&h->red, 5 * sizeof(double)))) {
+        (rc = alloc((void**)&h->red, 5 * sizeof(double))) ||
+        (rc = alloc((void**)&h->bits, (size_t)h->words * (size_t)desc->ny * sizeof(uint32_t)))) {
         free_handle(h);
         return rc;
     }
@@ -329,6 +345,7 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
     // the reference's init pass.
     if (e == hipSuccess) e = hipMemsetAsync(h->buf[0], 0, h->pop_bytes(), h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(h->buf[1], 0, h->pop_bytes(), h->stream);
+    if (e == hipSuccess && rebuild_mask(h) != LBM_OK) e = hipErrorLaunchFailure;
     if (e != hipSuccess) {
         free_handle(h);
         return fail(LBM_ERR_HIP, "stream/event setup: %s", hipGetErrorString(e));
@@ -417,13 +434,14 @@ int lbm_write_populations(lbm_handle* h, int buf, const float* f)
     if (buf != 0 && buf != 1) return fail(LBM_ERR_INVALID, "buffer must be 0 or 1");
     if (!f) return fail(LBM_ERR_INVALID, "null populations");
     const size_t nb = (size_t)h->n * sizeof(float);
-    if (!h->f16) {
+    if (!h->physics()) {
+        // Parity mode keeps the reference's buffer contents: f itself.
         HIP_TRY(hipMemcpy2DAsync(h->buf[buf], (size_t)h->plane * 4, f, nb, nb, 9, hipMemcpyHostToDevice, h->stream));
     } else {
         float* stage = nullptr;
         HIP_TRY(hipMalloc(&stage, 9 * nb));
         hipError_t e = hipMemcpyAsync(stage, f, 9 * nb, hipMemcpyHostToDevice, h->stream);
-        if (e == hipSuccess) e = launch_f32_to_dev16(stage, h->buf[buf], h->n, h->plane, h->scale, h->stream);
+        if (e == hipSuccess) e = launch_host_to_dev(h->f16, stage, h->buf[buf], h->n, h->plane, h->scale, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         (void)hipFree(stage);
         if (e != hipSuccess) return fail(LBM_ERR_HIP, "population upload: %s", hipGetErrorString(e));
@@ -441,13 +459,13 @@ int lbm_read_populations(lbm_handle* h, int buf, float* f)
     if (buf != 0 && buf != 1) return fail(LBM_ERR_INVALID, "buffer must be 0 or 1");
     if (!f) return fail(LBM_ERR_INVALID, "null populations");
     const size_t nb = (size_t)h->n * sizeof(float);
-    if (!h->f16) {
+    if (!h->physics()) {
         HIP_TRY(hipMemcpy2DAsync(f, nb, h->buf[buf], (size_t)h->plane * 4, nb, 9, hipMemcpyDeviceToHost, h->stream));
         HIP_TRY(hipStreamSynchronize(h->stream));
     } else {
         float* stage = nullptr;
         HIP_TRY(hipMalloc(&stage, 9 * nb));
-        hipError_t e = launch_dev16_to_f32(h->buf[buf], stage, h->n, h->plane, h->inv_scale, h->stream);
+        hipError_t e = launch_dev_to_host(h->f16, h->buf[buf], stage, h->n, h->plane, h->inv_scale, h->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(f, stage, 9 * nb, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         (void)hipFree(stage);
@@ -643,13 +661,13 @@ int lbm_autotune(lbm_handle* h, int steps, int* best_variant, double* ms_per_ste
     for (int v = 0; v < kNumStepVariants && e == hipSuccess; ++v) {
         for (int s = 0; s < 2 && e == hipSuccess; ++s) {
             e = launch_phys_step(v, h->f16, h->d.x_edge, h->force(), false, h->phys(), h->buf[h->cur],
-                                 h->buf[h->cur ^ 1], h->mat, h->macro, h->stream);
+                                 h->buf[h->cur ^ 1], h->bits, h->words, h->macro, h->stream);
             h->cur ^= 1;
         }
         if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
         for (int s = 0; s < steps && e == hipSuccess; ++s) {
             e = launch_phys_step(v, h->f16, h->d.x_edge, h->force(), false, h->phys(), h->buf[h->cur],
-                                 h->buf[h->cur ^ 1], h->mat, h->macro, h->stream);
+                                 h->buf[h->cur ^ 1], h->bits, h->words, h->macro, h->stream);
             h->cur ^= 1;
         }
         if (e == hipSuccess) e = hipEventRecord(h->ev1, h->stream);
@@ -670,6 +688,35 @@ int lbm_autotune(lbm_handle* h, int steps, int* best_variant, double* ms_per_ste
     h->variant = best;
     if (best_variant) *best_variant = best;
     if (ms_per_step) *ms_per_step = best_ms;
+    return LBM_OK;
+}
+
+int lbm_state_size(lbm_handle* h, size_t* bytes)
+{
+    if (!h || !bytes) return fail(LBM_ERR_INVALID, "null argument");
+    *bytes = h->pop_bytes();
+    return LBM_OK;
+}
+
+int lbm_save_state(lbm_handle* h, void* out, size_t bytes)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!out || bytes != h->pop_bytes())
+        return fail(LBM_ERR_INVALID, "state buffer must be %zu bytes", h->pop_bytes());
+    HIP_TRY(hipMemcpyAsync(out, h->buf[h->cur], bytes, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+int lbm_load_state(lbm_handle* h, const void* in, size_t bytes)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!in || bytes != h->pop_bytes())
+        return fail(LBM_ERR_INVALID, "state buffer must be %zu bytes", h->pop_bytes());
+    HIP_TRY(hipMemcpyAsync(h->buf[h->cur], in, bytes, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
     return LBM_OK;
 }
 

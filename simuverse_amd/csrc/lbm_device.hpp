@@ -45,9 +45,10 @@ __host__ __device__ constexpr float max_wgsl(int i)
 __host__ __device__ inline bool mat_solid(uint32_t m) { return m == 2u || m == 4u; }
 __host__ __device__ inline bool mat_accel(uint32_t m) { return m == 3u || m == 6u; }
 
-// Population storage.  Kernels work on deviations g_i = f_i - w_i from the
-// rest state; the storage decides how a deviation is kept in memory.
-//   float     stores f_i itself (binary32)
+// Population storage of the physics mode.  Kernels work on deviations
+// g_i = f_i - w_i from the rest state, and memory holds the deviation too, so
+// the storage rounding is relative to g (~1e-4) rather than to f (~0.1):
+//   float     stores f_i - w_i as binary32
 //   _Float16  stores (f_i - w_i) * 2^shift as binary16
 template <typename T>
 struct Store;
@@ -55,18 +56,20 @@ struct Store;
 template <>
 struct Store<float> {
     static constexpr int bytes = 4;
-    __device__ static inline float load_dev(const float* p, uint32_t k, float w, float /*inv_scale*/)
+    __device__ static inline float load_dev(const float* p, uint32_t k, float /*w*/, float /*inv_scale*/)
     {
-        return p[k] - w;
+        return p[k];
     }
-    __device__ static inline void store_dev(float* p, uint32_t k, float g, float w, float /*scale*/)
+    __device__ static inline void store_dev(float* p, uint32_t k, float g, float /*w*/, float /*scale*/)
     {
-        p[k] = w + g;
+        p[k] = g;
     }
-    __device__ static inline double load_f64(const float* p, uint32_t k, float /*w*/, float /*inv_scale*/)
+    __device__ static inline double load_f64(const float* p, uint32_t k, float w, float /*inv_scale*/)
     {
-        return (double)p[k];
+        return (double)w + (double)p[k];
     }
+    __device__ static inline float to_storage(float g, float /*scale*/) { return g; }
+    __device__ static inline float from_storage(float s, float /*inv_scale*/) { return s; }
 };
 
 template <>
@@ -84,6 +87,8 @@ struct Store<_Float16> {
     {
         return (double)w + (double)((float)p[k] * inv_scale);
     }
+    __device__ static inline _Float16 to_storage(float g, float scale) { return (_Float16)(g * scale); }
+    __device__ static inline float from_storage(_Float16 s, float inv_scale) { return (float)s * inv_scale; }
 };
 
 // Arguments shared by the physics-mode kernels of one domain.

@@ -15,14 +15,19 @@ struct StepVariant {
 extern const StepVariant kStepVariants[];
 extern const int kNumStepVariants;
 
+// Words per row of the packed solid mask (bit x+1 of a row is cell x).
+inline uint32_t solid_mask_words(int nx) { return (((uint32_t)nx + 2u) / 32u + 2u + 1u) & ~1u; }
+hipError_t launch_build_solid_bits(const uint8_t* mat, int nx, int ny, int periodic_x, uint32_t words,
+                                  uint32_t* bits, hipStream_t s);
 hipError_t launch_phys_step(int variant, bool f16, int xedge, bool force, bool export_macro, const PhysArgs& a,
-                            const void* src, void* dst, const uint8_t* solid, float* macro, hipStream_t s);
+                            const void* src, void* dst, const uint32_t* bits, uint32_t words, float* macro,
+                            hipStream_t s);
 hipError_t launch_phys_macro(bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
-                             const uint8_t* solid, float* rho, float* ux, float* uy, hipStream_t s);
-hipError_t launch_phys_prime(bool f16, int xedge, const PhysArgs& a, void* buf, const uint8_t* solid,
+                             const uint8_t* mat, float* rho, float* ux, float* uy, hipStream_t s);
+hipError_t launch_phys_prime(bool f16, int xedge, const PhysArgs& a, void* buf, const uint8_t* mat,
                              hipStream_t s);
-hipError_t launch_phys_fill(bool f16, const PhysArgs& a, void* buf, const uint8_t* solid, hipStream_t s);
-hipError_t launch_phys_reduce(bool f16, int xedge, const PhysArgs& a, const void* src, const uint8_t* solid,
+hipError_t launch_phys_fill(bool f16, const PhysArgs& a, void* buf, const uint8_t* mat, hipStream_t s);
+hipError_t launch_phys_reduce(bool f16, int xedge, const PhysArgs& a, const void* src, const uint8_t* mat,
                               double* partial, int nblocks, double* out, hipStream_t s);
 
 hipError_t launch_wgsl_step(bool export_macro, const WgslArgs& a, const float* src, float* dst, uint8_t* mat,
@@ -34,9 +39,12 @@ hipError_t launch_wgsl_reset(const WgslArgs& a, int fluid_ty, float* A, float* B
 hipError_t launch_wgsl_reduce(const WgslArgs& a, const float* src, const uint8_t* mat, double* partial,
                               int nblocks, double* out, hipStream_t s);
 
-hipError_t launch_f32_to_dev16(const float* in, void* out, uint32_t n, uint32_t plane, float scale, hipStream_t s);
-hipError_t launch_dev16_to_f32(const void* in, float* out, uint32_t n, uint32_t plane, float inv_scale,
-                               hipStream_t s);
+// Host-format populations (dense binary32 f) <-> physics-mode device planes
+// of deviations (binary32, or binary16 scaled by 2^shift).
+hipError_t launch_host_to_dev(bool f16, const float* in, void* out, uint32_t n, uint32_t plane, float scale,
+                              hipStream_t s);
+hipError_t launch_dev_to_host(bool f16, const void* in, float* out, uint32_t n, uint32_t plane, float inv_scale,
+                              hipStream_t s);
 hipError_t launch_pack_rgba16f(const float* macro, uint32_t n, uint16_t* out, hipStream_t s);
 hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s);
 

@@ -31,11 +31,11 @@ namespace lbm {
 // inlet on column 0.
 template <typename T, int XEDGE>
 __device__ inline void phys_gather(const PhysArgs& a, const T* __restrict__ src,
-                                   const uint8_t* __restrict__ solid, int x, int y, float g[9])
+                                   const uint8_t* __restrict__ mat, int x, int y, float g[9])
 {
     const int nx = a.nx, ny = a.ny;
     int cx = x;
-    if (XEDGE == 1 && x == nx - 1 && !solid[(uint32_t)y * (uint32_t)nx + (uint32_t)(nx - 2)]) cx = nx - 2;
+    if (XEDGE == 1 && x == nx - 1 && !mat_solid(mat[(uint32_t)y * (uint32_t)nx + (uint32_t)(nx - 2)])) cx = nx - 2;
     const int ym = (y == 0) ? ny - 1 : y - 1;
     const int yp = (y == ny - 1) ? 0 : y + 1;
     int xm = cx - 1, xp = cx + 1;
@@ -113,15 +113,33 @@ __device__ inline void phys_collide(const PhysArgs& a, float g[9], float rho, fl
     }
 }
 
+// Packed solid mask: row y holds bit x+1 for cell x; bit 0 and bit nx+1
+// are the x-neighbours beyond the edges (the wrapped cells when x is
+// periodic, empty otherwise).  `words` per row is even and covers bit
+// nx+2, so the 64-bit window read below never leaves the row.
+// Returns bits (x-1, x, x+1) of row y as bits 0, 1, 2.
+__device__ inline uint32_t solid3(const uint32_t* __restrict__ bits, uint32_t words, int y, int x)
+{
+    const uint32_t* p = bits + (uint32_t)y * words + ((uint32_t)x >> 5);
+    const uint64_t v = (uint64_t)p[0] | ((uint64_t)p[1] << 32);
+    return (uint32_t)(v >> ((uint32_t)x & 31u)) & 7u;
+}
+
 template <typename T, int XEDGE, bool FORCE, bool EXPORT>
 __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T* __restrict__ dst,
-                                 const uint8_t* __restrict__ solid, float* __restrict__ macro,
-                                 uint32_t n, int x, int y)
+                                 const uint32_t* __restrict__ bits, uint32_t words,
+                                 float* __restrict__ macro, uint32_t n, int x, int y)
 {
     const int nx = a.nx, ny = a.ny;
     const uint32_t r0 = (uint32_t)y * (uint32_t)nx;
     const uint32_t c = r0 + (uint32_t)x;
-    if (solid[c]) {
+    const int ym = (y == 0) ? ny - 1 : y - 1;
+    const int yp = (y == ny - 1) ? 0 : y + 1;
+    // Solid flags of the 3x3 neighbourhood: three 64-bit windows of the mask.
+    const uint32_t sm = solid3(bits, words, ym, x);
+    const uint32_t s0 = solid3(bits, words, y, x);
+    const uint32_t sp = solid3(bits, words, yp, x);
+    if (s0 & 2u) {
         if (EXPORT) {
             macro[c] = 0.0f;
             macro[c + n] = 0.0f;
@@ -131,7 +149,38 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
         return;
     }
     float g[9];
-    phys_gather<T, XEDGE>(a, src, solid, x, y, g);
+    {
+        // Gather (see phys_gather); the outlet test uses the west bit.
+        int cx = x;
+        if (XEDGE == 1 && x == nx - 1 && !(s0 & 1u)) cx = nx - 2;
+        int xm = cx - 1, xp = cx + 1;
+        if (XEDGE == 0) {
+            if (xm < 0) xm = nx - 1;
+            if (xp >= nx) xp = 0;
+        }
+        const uint32_t rm = (uint32_t)ym * (uint32_t)nx, rp = (uint32_t)yp * (uint32_t)nx;
+        const uint32_t cc = r0 + (uint32_t)cx;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const int ex = ex_of(i), ey = ey_of(i);
+            const int sx = ex == 1 ? xm : (ex == -1 ? xp : cx);
+            const uint32_t srow = ey == 1 ? rm : (ey == -1 ? rp : r0);
+            bool oob = false;
+            if (XEDGE == 1 && ex != 0) oob = (sx < 0) || (sx >= nx);
+            const uint32_t k = oob ? cc + (uint32_t)inv_of(i) * a.plane : srow + (uint32_t)sx + (uint32_t)i * a.plane;
+            g[i] = Store<T>::load_dev(src, k, w_phys(i), a.inv_scale);
+        }
+        if (XEDGE == 1 && x == 0) {
+            const float u0 = a.inlet_ux;
+            const float G = g[0] + g[2] + g[4] + 2.0f * (g[3] + g[6] + g[7]);
+            const float rho = (1.0f + G) / (1.0f - u0);
+            const float ru = rho * u0;
+            const float h = 0.5f * (g[4] - g[2]);
+            g[1] = g[3] + (2.0f / 3.0f) * ru;
+            g[5] = g[7] + h + (1.0f / 6.0f) * ru;
+            g[8] = g[6] - h + (1.0f / 6.0f) * ru;
+        }
+    }
     float rho, ux, uy;
     phys_moments<FORCE>(a, g, rho, ux, uy);
     if (EXPORT) {
@@ -142,9 +191,11 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
     }
     phys_collide<FORCE>(a, g, rho, ux, uy);
 
-    // Neighbour coordinates for the push (x + e_i).
-    const int ym = (y == 0) ? ny - 1 : y - 1;
-    const int yp = (y == ny - 1) ? 0 : y + 1;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Store<T>::store_dev(dst, c + (uint32_t)i * a.plane, g[i], w_phys(i), a.scale);
+    // Pushes into solid neighbours x + e_i (slot inv(i)); the mask's edge
+    // bits are empty for a non-periodic x, so nothing is pushed outside.
+    if ((sm | sp | (s0 & 5u)) == 0u) return;
     int xm = x - 1, xp = x + 1;
     if (XEDGE == 0) {
         if (xm < 0) xm = nx - 1;
@@ -152,15 +203,13 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
     }
     const uint32_t rm = (uint32_t)ym * (uint32_t)nx, rp = (uint32_t)yp * (uint32_t)nx;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        Store<T>::store_dev(dst, c + (uint32_t)i * a.plane, g[i], w_phys(i), a.scale);
-        if (i == 0) continue;
+    for (int i = 1; i < 9; ++i) {
         const int ex = ex_of(i), ey = ey_of(i);
+        const uint32_t srow = ey == 1 ? sp : (ey == -1 ? sm : s0);
+        if (!((srow >> (1 + ex)) & 1u)) continue;
         const int tx = ex == 1 ? xp : (ex == -1 ? xm : x);
         const uint32_t trow = ey == 1 ? rp : (ey == -1 ? rm : r0);
-        if (XEDGE == 1 && ex != 0 && (tx < 0 || tx >= nx)) continue;
-        const uint32_t t = trow + (uint32_t)tx;
-        if (solid[t]) Store<T>::store_dev(dst, t + (uint32_t)inv_of(i) * a.plane, g[i], w_phys(i), a.scale);
+        Store<T>::store_dev(dst, trow + (uint32_t)tx + (uint32_t)inv_of(i) * a.plane, g[i], w_phys(i), a.scale);
     }
 }
 
@@ -169,8 +218,8 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
 // instruction of a wavefront covers 64 consecutive cells of one plane.
 template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS, int CPT>
 __global__ __launch_bounds__(BS) void k_phys_step(PhysArgs a, const T* __restrict__ src,
-                                                  T* __restrict__ dst, const uint8_t* __restrict__ solid,
-                                                  float* __restrict__ macro, uint32_t nseg)
+                                                  T* __restrict__ dst, const uint32_t* __restrict__ bits,
+                                                  uint32_t words, float* __restrict__ macro, uint32_t nseg)
 {
     const uint32_t total = nseg * (uint32_t)a.ny;
     const uint32_t tile = xcd_remap(blockIdx.x, total);
@@ -180,13 +229,35 @@ __global__ __launch_bounds__(BS) void k_phys_step(PhysArgs a, const T* __restric
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int x = x0 + k * BS;
-        if (x < a.nx) phys_cell<T, XEDGE, FORCE, EXPORT>(a, src, dst, solid, macro, n, x, y);
+        if (x < a.nx) phys_cell<T, XEDGE, FORCE, EXPORT>(a, src, dst, bits, words, macro, n, x, y);
     }
+}
+
+// Builds the packed solid mask from the material bytes (one 32-bit word per
+// thread).
+__global__ __launch_bounds__(256) void k_build_solid_bits(const uint8_t* __restrict__ mat, int nx, int ny,
+                                                          int periodic_x, uint32_t words,
+                                                          uint32_t* __restrict__ bits)
+{
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= words * (uint32_t)ny) return;
+    const uint32_t y = k / words, w = k % words;
+    uint32_t v = 0;
+    for (int b = 0; b < 32; ++b) {
+        int x = (int)(32u * w + (uint32_t)b) - 1;
+        if (x < -1 || x > nx) continue;
+        if (x == -1 || x == nx) {
+            if (!periodic_x) continue;
+            x = (x == -1) ? nx - 1 : 0;
+        }
+        if (mat_solid(mat[y * (uint32_t)nx + (uint32_t)x])) v |= 1u << b;
+    }
+    bits[k] = v;
 }
 
 template <typename T, int XEDGE, bool FORCE>
 __global__ __launch_bounds__(256) void k_phys_macro(PhysArgs a, const T* __restrict__ src,
-                                                    const uint8_t* __restrict__ solid,
+                                                    const uint8_t* __restrict__ mat,
                                                     float* __restrict__ rho_out, float* __restrict__ ux_out,
                                                     float* __restrict__ uy_out)
 {
@@ -194,12 +265,12 @@ __global__ __launch_bounds__(256) void k_phys_macro(PhysArgs a, const T* __restr
     const uint32_t c = blockIdx.x * 256u + threadIdx.x;
     if (c >= n) return;
     const int y = (int)(c / (uint32_t)a.nx), x = (int)(c % (uint32_t)a.nx);
-    if (solid[c]) {
+    if (mat_solid(mat[c])) {
         rho_out[c] = ux_out[c] = uy_out[c] = 0.0f;
         return;
     }
     float g[9], rho, ux, uy;
-    phys_gather<T, XEDGE>(a, src, solid, x, y, g);
+    phys_gather<T, XEDGE>(a, src, mat, x, y, g);
     phys_moments<FORCE>(a, g, rho, ux, uy);
     rho_out[c] = rho;
     ux_out[c] = ux;
@@ -211,11 +282,11 @@ __global__ __launch_bounds__(256) void k_phys_macro(PhysArgs a, const T* __restr
 // bounce-back values where the next gather expects them.
 template <typename T, int XEDGE>
 __global__ __launch_bounds__(256) void k_phys_prime(PhysArgs a, T* __restrict__ buf,
-                                                    const uint8_t* __restrict__ solid)
+                                                    const uint8_t* __restrict__ mat)
 {
     const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
     const uint32_t c = blockIdx.x * 256u + threadIdx.x;
-    if (c >= n || solid[c]) return;
+    if (c >= n || mat_solid(mat[c])) return;
     const int nx = a.nx, ny = a.ny;
     const int y = (int)(c / (uint32_t)nx), x = (int)(c % (uint32_t)nx);
 #pragma unroll
@@ -230,19 +301,19 @@ __global__ __launch_bounds__(256) void k_phys_prime(PhysArgs a, T* __restrict__ 
             continue;
         }
         const uint32_t t = (uint32_t)ty * (uint32_t)nx + (uint32_t)tx;
-        if (solid[t]) buf[t + (uint32_t)inv_of(i) * a.plane] = buf[c + (uint32_t)i * a.plane];
+        if (mat_solid(mat[t])) buf[t + (uint32_t)inv_of(i) * a.plane] = buf[c + (uint32_t)i * a.plane];
     }
 }
 
 // Rest state: f = w on fluid cells (deviation 0), raw 0 on solid cells.
 template <typename T>
 __global__ __launch_bounds__(256) void k_phys_fill(PhysArgs a, T* __restrict__ buf,
-                                                   const uint8_t* __restrict__ solid)
+                                                   const uint8_t* __restrict__ mat)
 {
     const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
     const uint32_t c = blockIdx.x * 256u + threadIdx.x;
     if (c >= n) return;
-    const bool s = solid[c] != 0;
+    const bool s = mat_solid(mat[c]) != 0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         if (s) buf[c + (uint32_t)i * a.plane] = (T)0;
@@ -462,17 +533,17 @@ __device__ inline void block_sum5(double v[5], double* __restrict__ partial)
 
 template <typename T, int XEDGE>
 __global__ __launch_bounds__(256) void k_phys_reduce(PhysArgs a, const T* __restrict__ src,
-                                                     const uint8_t* __restrict__ solid,
+                                                     const uint8_t* __restrict__ mat,
                                                      double* __restrict__ partial)
 {
     const int nx = a.nx, ny = a.ny;
     const uint32_t n = (uint32_t)nx * (uint32_t)ny;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (uint32_t c = blockIdx.x * 256u + threadIdx.x; c < n; c += gridDim.x * 256u) {
-        if (solid[c]) continue;
+        if (mat_solid(mat[c])) continue;
         const int y = (int)(c / (uint32_t)nx), x = (int)(c % (uint32_t)nx);
         int cx = x;
-        if (XEDGE == 1 && x == nx - 1 && !solid[c - 1]) cx = nx - 2;
+        if (XEDGE == 1 && x == nx - 1 && !mat_solid(mat[c - 1])) cx = nx - 2;
         const uint32_t cc = (uint32_t)y * (uint32_t)nx + (uint32_t)cx;
         double f[9];
 #pragma unroll
@@ -517,7 +588,7 @@ __global__ __launch_bounds__(256) void k_phys_reduce(PhysArgs a, const T* __rest
                 continue;
             }
             const uint32_t t = (uint32_t)ty * (uint32_t)nx + (uint32_t)tx;
-            if (!solid[t]) continue;
+            if (!mat_solid(mat[t])) continue;
             const double v = Store<T>::load_f64(src, t + (uint32_t)inv_of(i) * a.plane, w_phys(i), a.inv_scale);
             acc[3] += 2.0 * (double)ex_of(i) * v;
             acc[4] += 2.0 * (double)ey_of(i) * v;
@@ -589,24 +660,26 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__
 // ---------------------------------------------------------------------------
 
 // Dense host layout f[i*n + c] (binary32) <-> device planes of deviations.
-__global__ __launch_bounds__(256) void k_f32_to_dev16(const float* __restrict__ in, _Float16* __restrict__ out,
-                                                      uint32_t n, uint32_t plane, float scale)
+template <typename T>
+__global__ __launch_bounds__(256) void k_host_to_dev(const float* __restrict__ in, T* __restrict__ out, uint32_t n,
+                                                     uint32_t plane, float scale)
 {
     const uint32_t c = blockIdx.x * 256u + threadIdx.x;
     if (c >= n) return;
 #pragma unroll
     for (int i = 0; i < 9; ++i)
-        out[c + (uint32_t)i * plane] = (_Float16)((in[c + (uint32_t)i * n] - w_phys(i)) * scale);
+        out[c + (uint32_t)i * plane] = Store<T>::to_storage(in[c + (uint32_t)i * n] - w_phys(i), scale);
 }
 
-__global__ __launch_bounds__(256) void k_dev16_to_f32(const _Float16* __restrict__ in, float* __restrict__ out,
-                                                      uint32_t n, uint32_t plane, float inv_scale)
+template <typename T>
+__global__ __launch_bounds__(256) void k_dev_to_host(const T* __restrict__ in, float* __restrict__ out, uint32_t n,
+                                                     uint32_t plane, float inv_scale)
 {
     const uint32_t c = blockIdx.x * 256u + threadIdx.x;
     if (c >= n) return;
 #pragma unroll
     for (int i = 0; i < 9; ++i)
-        out[c + (uint32_t)i * n] = w_phys(i) + (float)in[c + (uint32_t)i * plane] * inv_scale;
+        out[c + (uint32_t)i * n] = w_phys(i) + Store<T>::from_storage(in[c + (uint32_t)i * plane], inv_scale);
 }
 
 // (ux, uy, rho, w) planes -> rgba16float texels, round to nearest even.
@@ -636,13 +709,13 @@ __global__ __launch_bounds__(256) void k_copy16(const float4* __restrict__ src, 
 static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
 template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS, int CPT>
-static hipError_t launch_phys_variant(const PhysArgs& a, const void* src, void* dst, const uint8_t* solid,
-                                      float* macro, hipStream_t s)
+static hipError_t launch_phys_variant(const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
+                                      uint32_t words, float* macro, hipStream_t s)
 {
     const uint32_t nseg = (uint32_t)((a.nx + BS * CPT - 1) / (BS * CPT));
     const uint32_t total = nseg * (uint32_t)a.ny;
     hipLaunchKernelGGL((k_phys_step<T, XEDGE, FORCE, EXPORT, BS, CPT>), dim3(total), dim3(BS), 0, s, a,
-                       (const T*)src, (T*)dst, solid, macro, nseg);
+                       (const T*)src, (T*)dst, bits, words, macro, nseg);
     return hipGetLastError();
 }
 
@@ -653,49 +726,58 @@ const StepVariant kStepVariants[] = {
 const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
 
 template <typename T, int XEDGE, bool FORCE, bool EXPORT>
-static hipError_t dispatch_variant(int v, const PhysArgs& a, const void* src, void* dst, const uint8_t* solid,
-                                   float* macro, hipStream_t s)
+static hipError_t dispatch_variant(int v, const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
+                                   uint32_t words, float* macro, hipStream_t s)
 {
     switch (v) {
-    case 0: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 1>(a, src, dst, solid, macro, s);
-    case 1: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 2>(a, src, dst, solid, macro, s);
-    case 2: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 4>(a, src, dst, solid, macro, s);
-    case 3: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 8>(a, src, dst, solid, macro, s);
-    case 4: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 128, 4>(a, src, dst, solid, macro, s);
-    case 5: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 512, 2>(a, src, dst, solid, macro, s);
-    case 6: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 512, 4>(a, src, dst, solid, macro, s);
-    case 7: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 1024, 4>(a, src, dst, solid, macro, s);
+    case 0: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 1>(a, src, dst, bits, words, macro, s);
+    case 1: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 2>(a, src, dst, bits, words, macro, s);
+    case 2: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 4>(a, src, dst, bits, words, macro, s);
+    case 3: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 8>(a, src, dst, bits, words, macro, s);
+    case 4: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 128, 4>(a, src, dst, bits, words, macro, s);
+    case 5: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 512, 2>(a, src, dst, bits, words, macro, s);
+    case 6: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 512, 4>(a, src, dst, bits, words, macro, s);
+    case 7: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 1024, 4>(a, src, dst, bits, words, macro, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <typename T, int XEDGE>
 static hipError_t dispatch_force(int v, bool force, bool exp, const PhysArgs& a, const void* src, void* dst,
-                                 const uint8_t* solid, float* macro, hipStream_t s)
+                                 const uint32_t* bits, uint32_t words, float* macro, hipStream_t s)
 {
     if (force)
-        return exp ? dispatch_variant<T, XEDGE, true, true>(v, a, src, dst, solid, macro, s)
-                   : dispatch_variant<T, XEDGE, true, false>(v, a, src, dst, solid, macro, s);
-    return exp ? dispatch_variant<T, XEDGE, false, true>(v, a, src, dst, solid, macro, s)
-               : dispatch_variant<T, XEDGE, false, false>(v, a, src, dst, solid, macro, s);
+        return exp ? dispatch_variant<T, XEDGE, true, true>(v, a, src, dst, bits, words, macro, s)
+                   : dispatch_variant<T, XEDGE, true, false>(v, a, src, dst, bits, words, macro, s);
+    return exp ? dispatch_variant<T, XEDGE, false, true>(v, a, src, dst, bits, words, macro, s)
+               : dispatch_variant<T, XEDGE, false, false>(v, a, src, dst, bits, words, macro, s);
 }
 
 hipError_t launch_phys_step(int variant, bool f16, int xedge, bool force, bool export_macro, const PhysArgs& a,
-                            const void* src, void* dst, const uint8_t* solid, float* macro, hipStream_t s)
+                            const void* src, void* dst, const uint32_t* bits, uint32_t words, float* macro,
+                            hipStream_t s)
 {
     if (f16)
-        return xedge ? dispatch_force<_Float16, 1>(variant, force, export_macro, a, src, dst, solid, macro, s)
-                     : dispatch_force<_Float16, 0>(variant, force, export_macro, a, src, dst, solid, macro, s);
-    return xedge ? dispatch_force<float, 1>(variant, force, export_macro, a, src, dst, solid, macro, s)
-                 : dispatch_force<float, 0>(variant, force, export_macro, a, src, dst, solid, macro, s);
+        return xedge ? dispatch_force<_Float16, 1>(variant, force, export_macro, a, src, dst, bits, words, macro, s)
+                     : dispatch_force<_Float16, 0>(variant, force, export_macro, a, src, dst, bits, words, macro, s);
+    return xedge ? dispatch_force<float, 1>(variant, force, export_macro, a, src, dst, bits, words, macro, s)
+                 : dispatch_force<float, 0>(variant, force, export_macro, a, src, dst, bits, words, macro, s);
+}
+
+hipError_t launch_build_solid_bits(const uint8_t* mat, int nx, int ny, int periodic_x, uint32_t words,
+                                  uint32_t* bits, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_build_solid_bits, dim3(blocks_for((size_t)words * ny, 256)), dim3(256), 0, s, mat, nx, ny,
+                       periodic_x, words, bits);
+    return hipGetLastError();
 }
 
 hipError_t launch_phys_macro(bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
-                             const uint8_t* solid, float* rho, float* ux, float* uy, hipStream_t s)
+                             const uint8_t* mat, float* rho, float* ux, float* uy, hipStream_t s)
 {
     const unsigned nb = blocks_for((size_t)a.nx * a.ny, 256);
 #define LBM_MACRO(T, XE, FO) \
-    hipLaunchKernelGGL((k_phys_macro<T, XE, FO>), dim3(nb), dim3(256), 0, s, a, (const T*)src, solid, rho, ux, uy)
+    hipLaunchKernelGGL((k_phys_macro<T, XE, FO>), dim3(nb), dim3(256), 0, s, a, (const T*)src, mat, rho, ux, uy)
     if (f16) {
         if (xedge) { if (force) LBM_MACRO(_Float16, 1, true); else LBM_MACRO(_Float16, 1, false); }
         else { if (force) LBM_MACRO(_Float16, 0, true); else LBM_MACRO(_Float16, 0, false); }
@@ -707,37 +789,37 @@ hipError_t launch_phys_macro(bool f16, int xedge, bool force, const PhysArgs& a,
     return hipGetLastError();
 }
 
-hipError_t launch_phys_prime(bool f16, int xedge, const PhysArgs& a, void* buf, const uint8_t* solid,
+hipError_t launch_phys_prime(bool f16, int xedge, const PhysArgs& a, void* buf, const uint8_t* mat,
                              hipStream_t s)
 {
     const unsigned nb = blocks_for((size_t)a.nx * a.ny, 256);
     if (f16) {
-        if (xedge) hipLaunchKernelGGL((k_phys_prime<_Float16, 1>), dim3(nb), dim3(256), 0, s, a, (_Float16*)buf, solid);
-        else hipLaunchKernelGGL((k_phys_prime<_Float16, 0>), dim3(nb), dim3(256), 0, s, a, (_Float16*)buf, solid);
+        if (xedge) hipLaunchKernelGGL((k_phys_prime<_Float16, 1>), dim3(nb), dim3(256), 0, s, a, (_Float16*)buf, mat);
+        else hipLaunchKernelGGL((k_phys_prime<_Float16, 0>), dim3(nb), dim3(256), 0, s, a, (_Float16*)buf, mat);
     } else {
-        if (xedge) hipLaunchKernelGGL((k_phys_prime<float, 1>), dim3(nb), dim3(256), 0, s, a, (float*)buf, solid);
-        else hipLaunchKernelGGL((k_phys_prime<float, 0>), dim3(nb), dim3(256), 0, s, a, (float*)buf, solid);
+        if (xedge) hipLaunchKernelGGL((k_phys_prime<float, 1>), dim3(nb), dim3(256), 0, s, a, (float*)buf, mat);
+        else hipLaunchKernelGGL((k_phys_prime<float, 0>), dim3(nb), dim3(256), 0, s, a, (float*)buf, mat);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_phys_fill(bool f16, const PhysArgs& a, void* buf, const uint8_t* solid, hipStream_t s)
+hipError_t launch_phys_fill(bool f16, const PhysArgs& a, void* buf, const uint8_t* mat, hipStream_t s)
 {
     const unsigned nb = blocks_for((size_t)a.nx * a.ny, 256);
-    if (f16) hipLaunchKernelGGL((k_phys_fill<_Float16>), dim3(nb), dim3(256), 0, s, a, (_Float16*)buf, solid);
-    else hipLaunchKernelGGL((k_phys_fill<float>), dim3(nb), dim3(256), 0, s, a, (float*)buf, solid);
+    if (f16) hipLaunchKernelGGL((k_phys_fill<_Float16>), dim3(nb), dim3(256), 0, s, a, (_Float16*)buf, mat);
+    else hipLaunchKernelGGL((k_phys_fill<float>), dim3(nb), dim3(256), 0, s, a, (float*)buf, mat);
     return hipGetLastError();
 }
 
-hipError_t launch_phys_reduce(bool f16, int xedge, const PhysArgs& a, const void* src, const uint8_t* solid,
+hipError_t launch_phys_reduce(bool f16, int xedge, const PhysArgs& a, const void* src, const uint8_t* mat,
                               double* partial, int nblocks, double* out, hipStream_t s)
 {
     if (f16) {
-        if (xedge) hipLaunchKernelGGL((k_phys_reduce<_Float16, 1>), dim3(nblocks), dim3(256), 0, s, a, (const _Float16*)src, solid, partial);
-        else hipLaunchKernelGGL((k_phys_reduce<_Float16, 0>), dim3(nblocks), dim3(256), 0, s, a, (const _Float16*)src, solid, partial);
+        if (xedge) hipLaunchKernelGGL((k_phys_reduce<_Float16, 1>), dim3(nblocks), dim3(256), 0, s, a, (const _Float16*)src, mat, partial);
+        else hipLaunchKernelGGL((k_phys_reduce<_Float16, 0>), dim3(nblocks), dim3(256), 0, s, a, (const _Float16*)src, mat, partial);
     } else {
-        if (xedge) hipLaunchKernelGGL((k_phys_reduce<float, 1>), dim3(nblocks), dim3(256), 0, s, a, (const float*)src, solid, partial);
-        else hipLaunchKernelGGL((k_phys_reduce<float, 0>), dim3(nblocks), dim3(256), 0, s, a, (const float*)src, solid, partial);
+        if (xedge) hipLaunchKernelGGL((k_phys_reduce<float, 1>), dim3(nblocks), dim3(256), 0, s, a, (const float*)src, mat, partial);
+        else hipLaunchKernelGGL((k_phys_reduce<float, 0>), dim3(nblocks), dim3(256), 0, s, a, (const float*)src, mat, partial);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -783,17 +865,27 @@ hipError_t launch_wgsl_reduce(const WgslArgs& a, const float* src, const uint8_t
     return hipGetLastError();
 }
 
-hipError_t launch_f32_to_dev16(const float* in, void* out, uint32_t n, uint32_t plane, float scale, hipStream_t s)
+hipError_t launch_host_to_dev(bool f16, const float* in, void* out, uint32_t n, uint32_t plane, float scale,
+                              hipStream_t s)
 {
-    hipLaunchKernelGGL(k_f32_to_dev16, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, (_Float16*)out, n, plane, scale);
+    if (f16)
+        hipLaunchKernelGGL(k_host_to_dev<_Float16>, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, (_Float16*)out, n,
+                           plane, scale);
+    else
+        hipLaunchKernelGGL(k_host_to_dev<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, (float*)out, n, plane,
+                           scale);
     return hipGetLastError();
 }
 
-hipError_t launch_dev16_to_f32(const void* in, float* out, uint32_t n, uint32_t plane, float inv_scale,
-                               hipStream_t s)
+hipError_t launch_dev_to_host(bool f16, const void* in, float* out, uint32_t n, uint32_t plane, float inv_scale,
+                              hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dev16_to_f32, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const _Float16*)in, out, n,
-                       plane, inv_scale);
+    if (f16)
+        hipLaunchKernelGGL(k_dev_to_host<_Float16>, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const _Float16*)in,
+                           out, n, plane, inv_scale);
+    else
+        hipLaunchKernelGGL(k_dev_to_host<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const float*)in, out, n,
+                           plane, inv_scale);
     return hipGetLastError();
 }
 

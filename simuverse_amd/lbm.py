@@ -71,6 +71,9 @@ _SIGNATURES = [
     ("lbm_reset", C.c_int, [_P]),
     ("lbm_write_populations", C.c_int, [_P, C.c_int, _P]),
     ("lbm_read_populations", C.c_int, [_P, C.c_int, _P]),
+    ("lbm_state_size", C.c_int, [_P, C.POINTER(C.c_size_t)]),
+    ("lbm_save_state", C.c_int, [_P, _P, C.c_size_t]),
+    ("lbm_load_state", C.c_int, [_P, _P, C.c_size_t]),
     ("lbm_set_tau", C.c_int, [_P, C.c_float]),
     ("lbm_step", C.c_int, [_P, C.c_int, C.c_int]),
     ("lbm_run", C.c_int, [_P, C.c_int, C.c_int]),
@@ -220,6 +223,17 @@ class Lattice:
         out = np.empty((9, self.ny, self.nx), dtype=np.float32)
         _check(self._lib.lbm_read_populations(self._h, self.current_buffer() if buf is None else buf, _ptr(out)))
         return out
+
+    def save_state(self) -> np.ndarray:
+        size = C.c_size_t()
+        _check(self._lib.lbm_state_size(self._h, C.byref(size)))
+        out = np.empty(size.value, np.uint8)
+        _check(self._lib.lbm_save_state(self._h, _ptr(out), out.size))
+        return out
+
+    def load_state(self, state: np.ndarray) -> None:
+        state = np.ascontiguousarray(state, dtype=np.uint8)
+        _check(self._lib.lbm_load_state(self._h, _ptr(state), state.size))
 
     def set_tau(self, tau: float) -> None:
         _check(self._lib.lbm_set_tau(self._h, tau))

@@ -215,11 +215,15 @@ def test_physics_periodic_disks_force_vs_fp64():
     st, lat = physics_pair(nx, ny, 0.55, scene.solid, f0, 1000, force=(1e-6, 0.0))
     e = macro_errors(st, lat, scene.solid)
     print("physics periodic-disk errors vs fp64:", e, flush=True)
-    assert e["rho_l2"] < 1e-6 and e["u_l2"] < 1e-4
+    # The force adds 1e-7 per step to deviations of ~1e-3, which binary32
+    # resolves to ~1e-4 relative: u carries that floor.
+    assert e["rho_l2"] < 1e-6 and e["u_l2"] < 2e-4
     r_gpu = lat.reduce()
     r_ref = st.reduce()
-    np.testing.assert_allclose(r_gpu[0], r_ref[0], rtol=1e-6)
-    np.testing.assert_allclose(r_gpu[1:], r_ref[1:], rtol=1e-3, atol=1e-9)
+    print("reduce gpu", r_gpu, "fp64", r_ref, flush=True)
+    np.testing.assert_allclose(r_gpu[0], r_ref[0], rtol=1e-7)
+    assert np.hypot(*(r_gpu[1:3] - r_ref[1:3])) < 1e-4 * np.hypot(*r_ref[1:3])
+    assert np.hypot(*(r_gpu[3:] - r_ref[3:])) < 1e-3 * np.hypot(*r_ref[3:])
 
 
 def test_physics_reduce_identities_small():
@@ -277,13 +281,21 @@ def test_checkpoint_resume_bit_identical():
     b.write_materials(W.materials_from_solid(solid))
     b.write_populations(f0, 0)
     b.run(77)
-    ck = b.read_populations()
+    ck = b.save_state()
+    fck = b.read_populations()
     c = L.Lattice(nx, ny, 0.6, x_edge=L.X_INLET_OUTLET, inlet_ux=0.05)
     c.write_materials(W.materials_from_solid(solid))
-    c.write_populations(ck, 0)
+    c.load_state(ck)
     c.run(73)
+    assert np.array_equal(c.read_populations(), ref)
+    # Through the f-based exchange the state is rounded once (f = w + g in
+    # binary32), so the resumed run stays within a few ulp.
+    d = L.Lattice(nx, ny, 0.6, x_edge=L.X_INLET_OUTLET, inlet_ux=0.05)
+    d.write_materials(W.materials_from_solid(solid))
+    d.write_populations(fck, 0)
+    d.run(73)
     fl = ~solid
-    assert np.array_equal(c.read_populations()[:, fl], ref[:, fl])
+    np.testing.assert_allclose(d.read_populations()[:, fl], ref[:, fl], rtol=0, atol=1e-6)
 
 
 def test_f16_deviation_storage_vs_fp64():
