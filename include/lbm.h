@@ -219,7 +219,9 @@ int lbm_reduce(lbm_handle* h, double out[5]);
 /* Physics-mode step kernels are built for several tile shapes; every shape
  * computes bit-identical results. */
 int lbm_variant_count(void);
-int lbm_variant_info(int variant, int* block, int* cells_per_thread);
+/* block: threads per block; cells_per_thread: cells one thread updates;
+ * vec: consecutive cells per lane (1, or 4 with nx % 4 == 0). */
+int lbm_variant_info(int variant, int* block, int* cells_per_thread, int* vec);
 int lbm_set_variant(lbm_handle* h, int variant);
 int lbm_get_variant(lbm_handle* h, int* variant);
 /* Times every tile shape over `steps` steps on this handle's lattice and

@@ -603,11 +603,12 @@ int lbm_reduce(lbm_handle* h, double out[5])
 
 int lbm_variant_count(void) { return kNumStepVariants; }
 
-int lbm_variant_info(int variant, int* block, int* cells_per_thread)
+int lbm_variant_info(int variant, int* block, int* cells_per_thread, int* vec)
 {
     if (variant < 0 || variant >= kNumStepVariants) return fail(LBM_ERR_INVALID, "no variant %d", variant);
     if (block) *block = kStepVariants[variant].block;
     if (cells_per_thread) *cells_per_thread = kStepVariants[variant].cells_per_thread;
+    if (vec) *vec = kStepVariants[variant].vec;
     return LBM_OK;
 }
 
@@ -616,6 +617,8 @@ int lbm_set_variant(lbm_handle* h, int variant)
     int rc = check_handle(h);
     if (rc) return rc;
     if (variant < 0 || variant >= kNumStepVariants) return fail(LBM_ERR_INVALID, "no variant %d", variant);
+    if (!step_variant_ok(variant, h->d.nx))
+        return fail(LBM_ERR_INVALID, "variant %d needs nx divisible by %d", variant, kStepVariants[variant].vec);
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (variant != h->variant) drop_graphs(h);
     h->variant = variant;
@@ -659,6 +662,7 @@ int lbm_autotune(lbm_handle* h, int steps, int* best_variant, double* ms_per_ste
     int best = h->variant;
     double best_ms = 1e300;
     for (int v = 0; v < kNumStepVariants && e == hipSuccess; ++v) {
+        if (!step_variant_ok(v, h->d.nx)) continue;
         for (int s = 0; s < 2 && e == hipSuccess; ++s) {
             e = launch_phys_step(v, h->f16, h->d.x_edge, h->force(), false, h->phys(), h->buf[h->cur],
                                  h->buf[h->cur ^ 1], h->bits, h->words, h->macro, h->stream);

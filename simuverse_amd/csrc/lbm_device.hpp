@@ -70,6 +70,9 @@ struct Store<float> {
     }
     __device__ static inline float to_storage(float g, float /*scale*/) { return g; }
     __device__ static inline float from_storage(float s, float /*inv_scale*/) { return s; }
+    // Element already widened to binary32 <-> deviation.
+    __device__ static inline float scale_in(float v, float /*inv_scale*/) { return v; }
+    __device__ static inline float scale_out(float g, float /*scale*/) { return g; }
 };
 
 template <>
@@ -89,6 +92,8 @@ struct Store<_Float16> {
     }
     __device__ static inline _Float16 to_storage(float g, float scale) { return (_Float16)(g * scale); }
     __device__ static inline float from_storage(_Float16 s, float inv_scale) { return (float)s * inv_scale; }
+    __device__ static inline float scale_in(float v, float inv_scale) { return v * inv_scale; }
+    __device__ static inline float scale_out(float g, float scale) { return g * scale; }
 };
 
 // Arguments shared by the physics-mode kernels of one domain.

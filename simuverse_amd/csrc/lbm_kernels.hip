@@ -233,6 +233,161 @@ __global__ __launch_bounds__(BS) void k_phys_step(PhysArgs a, const T* __restric
     }
 }
 
+// Four consecutive cells of a row per lane.  Loads move 4 elements per
+// plane (16 B in binary32; 4 B-aligned for the x-shifted planes), stores
+// are aligned.  Only lanes whose 4 cells and their x-neighbours lie inside
+// the row (1 <= x0, x0 + 4 <= nx - 1) take this path; the per-cell
+// arithmetic is phys_moments/phys_collide, as in phys_cell, so results are
+// identical bit for bit.
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+    __device__ static inline void load(const float* p, float out[4])
+    {
+        float4 v;
+        __builtin_memcpy(&v, p, sizeof v);
+        out[0] = v.x;
+        out[1] = v.y;
+        out[2] = v.z;
+        out[3] = v.w;
+    }
+    __device__ static inline void store(float* p, const float in[4])
+    {
+        *reinterpret_cast<float4*>(p) = make_float4(in[0], in[1], in[2], in[3]);
+    }
+};
+template <>
+struct Vec4<_Float16> {
+    __device__ static inline void load(const _Float16* p, float out[4])
+    {
+        uint2 v;
+        __builtin_memcpy(&v, p, sizeof v);
+        out[0] = (float)__builtin_bit_cast(_Float16, (uint16_t)(v.x & 0xFFFFu));
+        out[1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(v.x >> 16));
+        out[2] = (float)__builtin_bit_cast(_Float16, (uint16_t)(v.y & 0xFFFFu));
+        out[3] = (float)__builtin_bit_cast(_Float16, (uint16_t)(v.y >> 16));
+    }
+    __device__ static inline void store(_Float16* p, const float in[4])
+    {
+        const uint32_t a = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)in[0]) |
+                           ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)in[1]) << 16);
+        const uint32_t b = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)in[2]) |
+                           ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)in[3]) << 16);
+        *reinterpret_cast<uint2*>(p) = make_uint2(a, b);
+    }
+};
+
+// Bits (x0-1 .. x0+4) of row y as bits 0..5 (x0 is a multiple of 4).
+__device__ inline uint32_t solid6(const uint32_t* __restrict__ bits, uint32_t words, int y, int x0)
+{
+    const uint32_t* p = bits + (uint32_t)y * words + ((uint32_t)x0 >> 5);
+    const uint64_t v = (uint64_t)p[0] | ((uint64_t)p[1] << 32);
+    return (uint32_t)(v >> ((uint32_t)x0 & 31u)) & 63u;
+}
+
+template <typename T, bool FORCE, bool EXPORT>
+__device__ inline void phys_cells4(const PhysArgs& a, const T* __restrict__ src, T* __restrict__ dst,
+                                   const uint32_t* __restrict__ bits, uint32_t words, float* __restrict__ macro,
+                                   uint32_t n, int x0, int y)
+{
+    const int nx = a.nx, ny = a.ny;
+    const int ym = (y == 0) ? ny - 1 : y - 1;
+    const int yp = (y == ny - 1) ? 0 : y + 1;
+    const uint32_t r0 = (uint32_t)y * (uint32_t)nx, rm = (uint32_t)ym * (uint32_t)nx,
+                   rp = (uint32_t)yp * (uint32_t)nx;
+    const uint32_t sm = solid6(bits, words, ym, x0);
+    const uint32_t s0 = solid6(bits, words, y, x0);
+    const uint32_t sp = solid6(bits, words, yp, x0);
+    const uint32_t self = (s0 >> 1) & 15u;  // bit k: cell x0 + k is solid
+    const uint32_t c0 = r0 + (uint32_t)x0;
+    if (self == 15u) {
+        if (EXPORT)
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) macro[c0 + k + q * n] = 0.0f;
+        return;
+    }
+    float g[9][4];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int ex = ex_of(i), ey = ey_of(i);
+        const uint32_t srow = ey == 1 ? rm : (ey == -1 ? rp : r0);
+        Vec4<T>::load(src + srow + (uint32_t)(x0 - ex) + (uint32_t)i * a.plane, g[i]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float gc[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) gc[i] = Store<T>::scale_in(g[i][k], a.inv_scale);
+        float rho, ux, uy;
+        phys_moments<FORCE>(a, gc, rho, ux, uy);
+        if (EXPORT) {
+            const bool s = (self >> k) & 1u;
+            macro[c0 + k] = s ? 0.0f : ux;
+            macro[c0 + k + n] = s ? 0.0f : uy;
+            macro[c0 + k + 2 * n] = s ? 0.0f : rho;
+            macro[c0 + k + 3 * n] = s ? 0.0f : 1.0f;
+        }
+        phys_collide<FORCE>(a, gc, rho, ux, uy);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) g[i][k] = gc[i];
+    }
+    if (self == 0u) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            float o[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) o[k] = Store<T>::scale_out(g[i][k], a.scale);
+            Vec4<T>::store(dst + c0 + (uint32_t)i * a.plane, o);
+        }
+    } else {
+        // A solid cell's slots receive pushes from its neighbours during
+        // this step: write the fluid cells only.
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if ((self >> k) & 1u) continue;
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                Store<T>::store_dev(dst, c0 + (uint32_t)k + (uint32_t)i * a.plane, g[i][k], w_phys(i), a.scale);
+        }
+    }
+    if ((sm | sp | s0) == 0u) return;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if ((self >> k) & 1u) continue;
+#pragma unroll
+        for (int i = 1; i < 9; ++i) {
+            const int ex = ex_of(i), ey = ey_of(i);
+            const uint32_t srow = ey == 1 ? sp : (ey == -1 ? sm : s0);
+            if (!((srow >> (k + 1 + ex)) & 1u)) continue;
+            const uint32_t trow = ey == 1 ? rp : (ey == -1 ? rm : r0);
+            Store<T>::store_dev(dst, trow + (uint32_t)(x0 + k + ex) + (uint32_t)inv_of(i) * a.plane, g[i][k],
+                                w_phys(i), a.scale);
+        }
+    }
+}
+
+template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS>
+__global__ __launch_bounds__(BS) void k_phys_step4(PhysArgs a, const T* __restrict__ src, T* __restrict__ dst,
+                                                   const uint32_t* __restrict__ bits, uint32_t words,
+                                                   float* __restrict__ macro, uint32_t nseg)
+{
+    const uint32_t total = nseg * (uint32_t)a.ny;
+    const uint32_t tile = xcd_remap(blockIdx.x, total);
+    const int y = (int)(tile / nseg);
+    const int x0 = (int)(tile % nseg) * (BS * 4) + 4 * (int)threadIdx.x;
+    if (x0 >= a.nx) return;
+    const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
+    if (x0 >= 1 && x0 + 4 <= a.nx - 1) {
+        phys_cells4<T, FORCE, EXPORT>(a, src, dst, bits, words, macro, n, x0, y);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) phys_cell<T, XEDGE, FORCE, EXPORT>(a, src, dst, bits, words, macro, n, x0 + k, y);
+    }
+}
+
 // Builds the packed solid mask from the material bytes (one 32-bit word per
 // thread).
 __global__ __launch_bounds__(256) void k_build_solid_bits(const uint8_t* __restrict__ mat, int nx, int ny,
@@ -719,11 +874,31 @@ static hipError_t launch_phys_variant(const PhysArgs& a, const void* src, void* 
     return hipGetLastError();
 }
 
-// Tile shapes the step kernel is built for: (block size, cells per thread).
+template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS>
+static hipError_t launch_phys_variant4(const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
+                                       uint32_t words, float* macro, hipStream_t s)
+{
+    if (a.nx % 4 != 0) return hipErrorInvalidValue;
+    const uint32_t nseg = (uint32_t)((a.nx + BS * 4 - 1) / (BS * 4));
+    const uint32_t total = nseg * (uint32_t)a.ny;
+    hipLaunchKernelGGL((k_phys_step4<T, XEDGE, FORCE, EXPORT, BS>), dim3(total), dim3(BS), 0, s, a,
+                       (const T*)src, (T*)dst, bits, words, macro, nseg);
+    return hipGetLastError();
+}
+
+// Tile shapes the step kernel is built for: (block size, cells per thread,
+// consecutive cells per lane).  vec = 4 needs nx % 4 == 0.
 const StepVariant kStepVariants[] = {
-    {256, 1}, {256, 2}, {256, 4}, {256, 8}, {128, 4}, {512, 2}, {512, 4}, {1024, 4},
+    {256, 1, 1}, {256, 2, 1}, {256, 4, 1}, {256, 8, 1}, {128, 4, 1}, {512, 2, 1},
+    {512, 4, 1}, {1024, 4, 1}, {64, 4, 4},  {128, 4, 4}, {256, 4, 4}, {512, 4, 4},
 };
 const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
+
+bool step_variant_ok(int v, int nx)
+{
+    if (v < 0 || v >= kNumStepVariants) return false;
+    return kStepVariants[v].vec == 1 || nx % kStepVariants[v].vec == 0;
+}
 
 template <typename T, int XEDGE, bool FORCE, bool EXPORT>
 static hipError_t dispatch_variant(int v, const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
@@ -738,6 +913,10 @@ static hipError_t dispatch_variant(int v, const PhysArgs& a, const void* src, vo
     case 5: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 512, 2>(a, src, dst, bits, words, macro, s);
     case 6: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 512, 4>(a, src, dst, bits, words, macro, s);
     case 7: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 1024, 4>(a, src, dst, bits, words, macro, s);
+    case 8: return launch_phys_variant4<T, XEDGE, FORCE, EXPORT, 64>(a, src, dst, bits, words, macro, s);
+    case 9: return launch_phys_variant4<T, XEDGE, FORCE, EXPORT, 128>(a, src, dst, bits, words, macro, s);
+    case 10: return launch_phys_variant4<T, XEDGE, FORCE, EXPORT, 256>(a, src, dst, bits, words, macro, s);
+    case 11: return launch_phys_variant4<T, XEDGE, FORCE, EXPORT, 512>(a, src, dst, bits, words, macro, s);
     default: return hipErrorInvalidValue;
     }
 }

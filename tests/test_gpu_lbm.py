@@ -287,7 +287,10 @@ def test_checkpoint_resume_bit_identical():
     c.write_materials(W.materials_from_solid(solid))
     c.load_state(ck)
     c.run(73)
-    assert np.array_equal(c.read_populations(), ref)
+    # every slot a fluid cell can read; solid slots no fluid cell reads keep
+    # whatever each handle's reset left in its other buffer
+    vis = visible_mask(solid)
+    assert np.array_equal(c.read_populations()[vis], ref[vis])
     # Through the f-based exchange the state is rounded once (f = w + g in
     # binary32), so the resumed run stays within a few ulp.
     d = L.Lattice(nx, ny, 0.6, x_edge=L.X_INLET_OUTLET, inlet_ux=0.05)
