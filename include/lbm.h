@@ -235,6 +235,47 @@ int lbm_set_graph_chunk(lbm_handle* h, int chunk);
  * calibration run for byte counters. */
 int lbm_copy_bandwidth(size_t bytes, int reps, double* ms_per_copy);
 
+/* ---- batched domains ----------------------------------------------------
+ * No reference counterpart: simuverse runs one lattice per simulator
+ * (fluid_simulator.rs:26-133).  A batch holds many independent physics-mode
+ * domains of uneven sizes in two shared population buffers, each with its
+ * own tau, x edges, inlet velocity, body force and solids; one persistent
+ * kernel launch steps every domain (100-step HIP-graph chunks in
+ * lbm_batch_run).  Each domain evolves exactly as a single-domain handle
+ * with the same descriptor (bit for bit). */
+typedef struct lbm_batch lbm_batch;
+
+typedef struct lbm_domain_desc {
+    int32_t nx, ny;      /* nx a multiple of 4, >= 8; ny >= 3              */
+    float tau;
+    int32_t x_edge;      /* LBM_X_*                                        */
+    float inlet_ux;      /* LBM_X_INLET_OUTLET: Zou-He inlet velocity      */
+    float force_x;       /* Guo body force                                 */
+    float force_y;
+    int32_t reserved[5]; /* must be zero                                   */
+} lbm_domain_desc;
+
+/* storage: LBM_STORAGE_F32 or LBM_STORAGE_F16_DEV (shift 10).  Domains
+ * start at rest (f = w) with every cell LBM_MAT_BULK. */
+int lbm_batch_create(const lbm_domain_desc* domains, int count, int storage, lbm_batch** out);
+int lbm_batch_destroy(lbm_batch* b);
+int lbm_batch_cells(lbm_batch* b, uint64_t* cells);
+/* Whole-domain material map (nx*ny bytes, LBM_MAT_* numbering). */
+int lbm_batch_write_materials(lbm_batch* b, int domain, const uint8_t* material);
+int lbm_batch_reset(lbm_batch* b);
+int lbm_batch_write_populations(lbm_batch* b, int domain, const float* f);
+int lbm_batch_read_populations(lbm_batch* b, int domain, float* f);
+int lbm_batch_run(lbm_batch* b, int nsteps);
+int lbm_batch_run_timed(lbm_batch* b, int nsteps, double* ms);
+int lbm_batch_synchronize(lbm_batch* b);
+int lbm_batch_macro(lbm_batch* b, int domain, float* rho, float* ux, float* uy);
+int lbm_batch_reduce(lbm_batch* b, int domain, double out[5]);
+int lbm_batch_variant_count(void);
+int lbm_batch_set_variant(lbm_batch* b, int variant);
+int lbm_batch_get_variant(lbm_batch* b, int* variant);
+int lbm_batch_set_graph_chunk(lbm_batch* b, int chunk);
+int lbm_batch_autotune(lbm_batch* b, int steps, int* best_variant, double* ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,19 @@ bool is_gfx950(int dev)
 
 }  // namespace
 
+// Records the message of a failure for lbm_last_error(); used by every
+// source file of the library.
+int lbm_fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
 struct lbm_handle {
     lbm_desc d{};
     int dev = 0;

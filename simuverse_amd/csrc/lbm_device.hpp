@@ -108,6 +108,20 @@ struct PhysArgs {
     float inv_scale;  // 2^-shift
 };
 
+// One domain of a batch: its physics arguments and where its populations,
+// solid mask and chunks start.
+struct BatchDomainDev {
+    PhysArgs a;
+    uint32_t pop_off;   // element offset of the domain's plane 0 in a buffer
+    uint32_t bits_off;  // word offset of the domain's solid mask
+    uint32_t words;     // mask words per row
+    uint32_t chunk0;    // index of the domain's first chunk
+    uint32_t ncells;    // nx * ny
+    int32_t x_edge;     // 0 periodic, 1 inlet/outlet
+    int32_t force;      // non-zero when (gx, gy) != 0
+    uint32_t pad;
+};
+
 // Arguments of the parity-mode (reference semantics) kernels.
 struct WgslArgs {
     int32_t nx, ny;

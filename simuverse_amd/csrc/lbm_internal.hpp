@@ -50,6 +50,17 @@ hipError_t launch_dev_to_host(bool f16, const void* in, float* out, uint32_t n, 
 hipError_t launch_pack_rgba16f(const float* macro, uint32_t n, uint16_t* out, hipStream_t s);
 hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s);
 
+struct BatchVariant {
+    int block;           // threads per block
+    int vec;             // consecutive cells per lane
+    int blocks_per_cu;   // persistent grid = blocks_per_cu * CUs
+};
+extern const BatchVariant kBatchVariants[];
+extern const int kNumBatchVariants;
+hipError_t launch_batch_step(int variant, bool f16, const BatchDomainDev* doms, const uint16_t* chunk_dom,
+                             uint32_t nchunks, uint32_t grid, const void* src, void* dst, const uint32_t* bits,
+                             hipStream_t s);
+
 // Number of blocks of the binary64 reduction (partials = 5 doubles each).
 constexpr int kReduceBlocks = 1024;
 

@@ -388,6 +388,57 @@ __global__ __launch_bounds__(BS) void k_phys_step4(PhysArgs a, const T* __restri
     }
 }
 
+// ---------------------------------------------------------------------------
+// Batched domains: one persistent launch steps every domain.
+// ---------------------------------------------------------------------------
+
+// The cells of chunk t (BS*V consecutive cells of one domain, row major).
+template <typename T, int XEDGE, bool FORCE, int BS, int V>
+__device__ inline void batch_chunk(const BatchDomainDev& D, uint32_t t, const T* __restrict__ src,
+                                   T* __restrict__ dst, const uint32_t* __restrict__ bits)
+{
+    const uint32_t c = (t - D.chunk0) * (uint32_t)(BS * V) + threadIdx.x * (uint32_t)V;
+    if (c >= D.ncells) return;
+    const int nx = D.a.nx;
+    const int y = (int)(c / (uint32_t)nx), x0 = (int)(c % (uint32_t)nx);
+    const T* s = src + D.pop_off;
+    T* o = dst + D.pop_off;
+    const uint32_t* m = bits + D.bits_off;
+    if (V == 4) {
+        if (x0 >= 1 && x0 + 4 <= nx - 1) {
+            phys_cells4<T, FORCE, false>(D.a, s, o, m, D.words, nullptr, D.ncells, x0, y);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                phys_cell<T, XEDGE, FORCE, false>(D.a, s, o, m, D.words, nullptr, D.ncells, x0 + k, y);
+        }
+    } else {
+        phys_cell<T, XEDGE, FORCE, false>(D.a, s, o, m, D.words, nullptr, D.ncells, x0, y);
+    }
+}
+
+// Block b (after the XCD remap) steps chunks [b*per_block, (b+1)*per_block).
+template <typename T, int BS, int V>
+__global__ __launch_bounds__(BS) void k_batch_step(const BatchDomainDev* __restrict__ doms,
+                                                   const uint16_t* __restrict__ chunk_dom, uint32_t nchunks,
+                                                   uint32_t per_block, const T* __restrict__ src, T* __restrict__ dst,
+                                                   const uint32_t* __restrict__ bits)
+{
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t t0 = b * per_block;
+    const uint32_t t1 = min(t0 + per_block, nchunks);
+    for (uint32_t t = t0; t < t1; ++t) {
+        const BatchDomainDev D = doms[chunk_dom[t]];
+        if (D.x_edge) {
+            if (D.force) batch_chunk<T, 1, true, BS, V>(D, t, src, dst, bits);
+            else batch_chunk<T, 1, false, BS, V>(D, t, src, dst, bits);
+        } else {
+            if (D.force) batch_chunk<T, 0, true, BS, V>(D, t, src, dst, bits);
+            else batch_chunk<T, 0, false, BS, V>(D, t, src, dst, bits);
+        }
+    }
+}
+
 // Builds the packed solid mask from the material bytes (one 32-bit word per
 // thread).
 __global__ __launch_bounds__(256) void k_build_solid_bits(const uint8_t* __restrict__ mat, int nx, int ny,
@@ -941,6 +992,43 @@ hipError_t launch_phys_step(int variant, bool f16, int xedge, bool force, bool e
                      : dispatch_force<_Float16, 0>(variant, force, export_macro, a, src, dst, bits, words, macro, s);
     return xedge ? dispatch_force<float, 1>(variant, force, export_macro, a, src, dst, bits, words, macro, s)
                  : dispatch_force<float, 0>(variant, force, export_macro, a, src, dst, bits, words, macro, s);
+}
+
+// Batched step shapes: (threads per block, consecutive cells per lane,
+// persistent blocks per CU).  A chunk is block*vec cells.
+const BatchVariant kBatchVariants[] = {
+    {256, 1, 8}, {64, 4, 16}, {128, 4, 8}, {256, 4, 4}, {256, 4, 8}, {128, 4, 16},
+};
+const int kNumBatchVariants = (int)(sizeof(kBatchVariants) / sizeof(kBatchVariants[0]));
+
+template <typename T>
+static hipError_t launch_batch_typed(int v, const BatchDomainDev* doms, const uint16_t* chunk_dom, uint32_t nchunks,
+                                     uint32_t grid, const void* src, void* dst, const uint32_t* bits, hipStream_t s)
+{
+    const uint32_t per_block = (nchunks + grid - 1) / grid;
+#define LBM_BATCH(BS, V)                                                                                   \
+    hipLaunchKernelGGL((k_batch_step<T, BS, V>), dim3(grid), dim3(BS), 0, s, doms, chunk_dom, nchunks,  \
+                       per_block, (const T*)src, (T*)dst, bits)
+    switch (v) {
+    case 0: LBM_BATCH(256, 1); break;
+    case 1: LBM_BATCH(64, 4); break;
+    case 2: LBM_BATCH(128, 4); break;
+    case 3: LBM_BATCH(256, 4); break;
+    case 4: LBM_BATCH(256, 4); break;
+    case 5: LBM_BATCH(128, 4); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef LBM_BATCH
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_step(int variant, bool f16, const BatchDomainDev* doms, const uint16_t* chunk_dom,
+                             uint32_t nchunks, uint32_t grid, const void* src, void* dst, const uint32_t* bits,
+                             hipStream_t s)
+{
+    if (nchunks == 0) return hipSuccess;
+    return f16 ? launch_batch_typed<_Float16>(variant, doms, chunk_dom, nchunks, grid, src, dst, bits, s)
+               : launch_batch_typed<float>(variant, doms, chunk_dom, nchunks, grid, src, dst, bits, s);
 }
 
 hipError_t launch_build_solid_bits(const uint8_t* mat, int nx, int ny, int periodic_x, uint32_t words,

@@ -91,7 +91,37 @@ _SIGNATURES = [
     ("lbm_autotune", C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     ("lbm_set_graph_chunk", C.c_int, [_P, C.c_int]),
     ("lbm_copy_bandwidth", C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    ("lbm_batch_create", C.c_int, [_P, C.c_int, C.c_int, C.POINTER(_P)]),
+    ("lbm_batch_destroy", C.c_int, [_P]),
+    ("lbm_batch_cells", C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    ("lbm_batch_write_materials", C.c_int, [_P, C.c_int, _P]),
+    ("lbm_batch_reset", C.c_int, [_P]),
+    ("lbm_batch_write_populations", C.c_int, [_P, C.c_int, _P]),
+    ("lbm_batch_read_populations", C.c_int, [_P, C.c_int, _P]),
+    ("lbm_batch_run", C.c_int, [_P, C.c_int]),
+    ("lbm_batch_run_timed", C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
+    ("lbm_batch_synchronize", C.c_int, [_P]),
+    ("lbm_batch_macro", C.c_int, [_P, C.c_int, _P, _P, _P]),
+    ("lbm_batch_reduce", C.c_int, [_P, C.c_int, _P]),
+    ("lbm_batch_variant_count", C.c_int, []),
+    ("lbm_batch_set_variant", C.c_int, [_P, C.c_int]),
+    ("lbm_batch_get_variant", C.c_int, [_P, C.POINTER(C.c_int)]),
+    ("lbm_batch_set_graph_chunk", C.c_int, [_P, C.c_int]),
+    ("lbm_batch_autotune", C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
 ]
+
+
+class DomainDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("tau", C.c_float),
+        ("x_edge", C.c_int32),
+        ("inlet_ux", C.c_float),
+        ("force_x", C.c_float),
+        ("force_y", C.c_float),
+        ("reserved", C.c_int32 * 5),
+    ]
 
 _lib = None
 
@@ -295,6 +325,98 @@ class Lattice:
         out = np.zeros(5, np.float64)
         _check(self._lib.lbm_reduce(self._h, _ptr(out)))
         return out
+
+
+class Batch:
+    """Many independent physics-mode domains stepped by one launch."""
+
+    def __init__(self, domains: list[dict], storage: int = STORAGE_F32):
+        self._lib = load_library()
+        arr = (DomainDesc * len(domains))()
+        for k, d in enumerate(domains):
+            arr[k].nx, arr[k].ny, arr[k].tau = d["nx"], d["ny"], d["tau"]
+            arr[k].x_edge = d.get("x_edge", X_INLET_OUTLET)
+            arr[k].inlet_ux = d.get("inlet_ux", 0.0)
+            fx, fy = d.get("force", (0.0, 0.0))
+            arr[k].force_x, arr[k].force_y = fx, fy
+        self.shapes = [(d["ny"], d["nx"]) for d in domains]
+        h = C.c_void_p()
+        _check(self._lib.lbm_batch_create(arr, len(domains), storage, C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _check(self._lib.lbm_batch_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cells(self) -> int:
+        c = C.c_uint64()
+        _check(self._lib.lbm_batch_cells(self._h, C.byref(c)))
+        return c.value
+
+    def write_materials(self, d: int, material: np.ndarray) -> None:
+        m = np.ascontiguousarray(material, dtype=np.uint8).reshape(-1)
+        assert m.size == self.shapes[d][0] * self.shapes[d][1]
+        _check(self._lib.lbm_batch_write_materials(self._h, d, _ptr(m)))
+
+    def reset(self) -> None:
+        _check(self._lib.lbm_batch_reset(self._h))
+
+    def write_populations(self, d: int, f: np.ndarray) -> None:
+        f = np.ascontiguousarray(f, dtype=np.float32).reshape(9, *self.shapes[d])
+        _check(self._lib.lbm_batch_write_populations(self._h, d, _ptr(f)))
+
+    def read_populations(self, d: int) -> np.ndarray:
+        out = np.empty((9, *self.shapes[d]), np.float32)
+        _check(self._lib.lbm_batch_read_populations(self._h, d, _ptr(out)))
+        return out
+
+    def run(self, nsteps: int) -> None:
+        _check(self._lib.lbm_batch_run(self._h, nsteps))
+
+    def run_timed(self, nsteps: int) -> float:
+        ms = C.c_double()
+        _check(self._lib.lbm_batch_run_timed(self._h, nsteps, C.byref(ms)))
+        return ms.value
+
+    def synchronize(self) -> None:
+        _check(self._lib.lbm_batch_synchronize(self._h))
+
+    def macro(self, d: int):
+        rho, ux, uy = (np.empty(self.shapes[d], np.float32) for _ in range(3))
+        _check(self._lib.lbm_batch_macro(self._h, d, _ptr(rho), _ptr(ux), _ptr(uy)))
+        return rho, ux, uy
+
+    def reduce(self, d: int) -> np.ndarray:
+        out = np.zeros(5)
+        _check(self._lib.lbm_batch_reduce(self._h, d, _ptr(out)))
+        return out
+
+    def set_variant(self, v: int) -> None:
+        _check(self._lib.lbm_batch_set_variant(self._h, v))
+
+    def variant(self) -> int:
+        v = C.c_int()
+        _check(self._lib.lbm_batch_get_variant(self._h, C.byref(v)))
+        return v.value
+
+    def set_graph_chunk(self, chunk: int) -> None:
+        _check(self._lib.lbm_batch_set_graph_chunk(self._h, chunk))
+
+    def autotune(self, steps: int = 10) -> tuple[int, float]:
+        v, ms = C.c_int(), C.c_double()
+        _check(self._lib.lbm_batch_autotune(self._h, steps, C.byref(v), C.byref(ms)))
+        return v.value, ms.value
+
+
+def batch_variant_count() -> int:
+    return load_library().lbm_batch_variant_count()
 
 
 def exported_symbols() -> list[str]:

@@ -238,3 +238,45 @@ def periodic_disks(nx: int, ny: int, fraction: float = 0.05, rmin: int = 4, rmax
         cys = np.append(cys, cy)
         rs = np.append(rs, r)
     return DiskScene(nx, ny, solid, np.array(disks, np.int64).reshape(-1, 3), seed)
+
+
+@dataclass
+class BatchDomain:
+    nx: int
+    ny: int
+    tau: float
+    inlet_ux: float
+    solid: np.ndarray  # (ny, nx) bool: walls on rows 0, ny-1 and the disks
+    disks: np.ndarray  # (k, 3) int: cx, cy, r
+
+
+def batch_domains(count: int = 512, seed: int = SEED, size_min: int = 64, size_max: int = 1024,
+                  size_step: int = 32, max_disks: int = 8) -> list[BatchDomain]:
+    """Seeded uneven channel domains: nx and ny uniform over the multiples of
+    size_step in [size_min, size_max], tau ~ U(0.51, 1.0), inlet ux ~
+    U(0.01, 0.08), walls on rows 0 and ny-1, and 0..max_disks disks of radius
+    r in [2, max(2, min(nx, ny)/8)] placed away from the walls, the inlet and
+    the outlet (rejected when they overlap an earlier disk)."""
+    rng = _Stream(seed)
+    sizes = np.arange(size_min, size_max + 1, size_step)
+    out = []
+    for _ in range(count):
+        nx = int(sizes[rng.integers(0, len(sizes) - 1)[0]])
+        ny = int(sizes[rng.integers(0, len(sizes) - 1)[0]])
+        tau = float(0.51 + 0.49 * rng.uniform()[0])
+        ux = float(0.01 + 0.07 * rng.uniform()[0])
+        k = int(rng.integers(0, max_disks)[0])
+        rmax = max(2, min(nx, ny) // 8)
+        y, x = np.mgrid[0:ny, 0:nx]
+        solid = (y == 0) | (y == ny - 1)
+        disks = []
+        for _ in range(k):
+            r = int(rng.integers(2, rmax)[0])
+            cx = int(rng.integers(r + 4, nx - r - 5)[0])
+            cy = int(rng.integers(r + 2, ny - r - 3)[0])
+            if any((cx - a) ** 2 + (cy - b) ** 2 <= (r + c + 1) ** 2 for a, b, c in disks):
+                continue
+            disks.append((cx, cy, r))
+            solid |= (x - cx) ** 2 + (y - cy) ** 2 <= r * r
+        out.append(BatchDomain(nx, ny, tau, ux, solid, np.array(disks, np.int64).reshape(-1, 3)))
+    return out
