@@ -241,7 +241,7 @@ def main():
             "solid_fraction": round(scene.solid_fraction, 5),
             "disks": int(len(scene.disks)),
             "seed": W.SEED,
-            "tile_variant": dict(zip(("block", "cells_per_thread", "vec"), L.variant_shapes()[variant])),
+            "tile_variant": dict(zip(("block", "cells_per_thread", "vec", "flags"), L.variant_shapes()[variant])),
             "graph_chunk": args.graph_chunk,
             "wall_ms": round(wall * 1e3, 3),
         },

@@ -220,8 +220,10 @@ int lbm_reduce(lbm_handle* h, double out[5]);
  * computes bit-identical results. */
 int lbm_variant_count(void);
 /* block: threads per block; cells_per_thread: cells one thread updates;
- * vec: consecutive cells per lane (1, or 4 with nx % 4 == 0). */
-int lbm_variant_info(int variant, int* block, int* cells_per_thread, int* vec);
+ * vec: consecutive cells per lane (1, or 4 with nx % 4 == 0); flags: bit 0
+ * non-temporal population stores, bit 1 non-temporal loads, bit 2 rows
+ * grouped per XCD. */
+int lbm_variant_info(int variant, int* block, int* cells_per_thread, int* vec, int* flags);
 int lbm_set_variant(lbm_handle* h, int variant);
 int lbm_get_variant(lbm_handle* h, int* variant);
 /* Times every tile shape over `steps` steps on this handle's lattice and

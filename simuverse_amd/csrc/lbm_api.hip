@@ -616,12 +616,13 @@ int lbm_reduce(lbm_handle* h, double out[5])
 
 int lbm_variant_count(void) { return kNumStepVariants; }
 
-int lbm_variant_info(int variant, int* block, int* cells_per_thread, int* vec)
+int lbm_variant_info(int variant, int* block, int* cells_per_thread, int* vec, int* flags)
 {
     if (variant < 0 || variant >= kNumStepVariants) return fail(LBM_ERR_INVALID, "no variant %d", variant);
     if (block) *block = kStepVariants[variant].block;
     if (cells_per_thread) *cells_per_thread = kStepVariants[variant].cells_per_thread;
     if (vec) *vec = kStepVariants[variant].vec;
+    if (flags) *flags = kStepVariants[variant].flags;
     return LBM_OK;
 }
 

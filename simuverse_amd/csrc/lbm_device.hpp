@@ -45,6 +45,28 @@ __host__ __device__ constexpr float max_wgsl(int i)
 __host__ __device__ inline bool mat_solid(uint32_t m) { return m == 2u || m == 4u; }
 __host__ __device__ inline bool mat_accel(uint32_t m) { return m == 3u || m == 6u; }
 
+// Loads and stores with an optional non-temporal hint (streamed data that
+// the next step reads back from HBM anyway).
+template <bool NT, typename T>
+__device__ inline T ld(const T* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename T>
+__device__ inline void st(T* p, T v)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// Step-kernel tuning flags.
+enum : int {
+    kNtStore = 1,  // non-temporal population stores
+    kNtLoad = 2,   // non-temporal population loads
+    kXcdRemap = 4  // give each XCD a contiguous range of rows
+};
+
 // Population storage of the physics mode.  Kernels work on deviations
 // g_i = f_i - w_i from the rest state, and memory holds the deviation too, so
 // the storage rounding is relative to g (~1e-4) rather than to f (~0.1):

@@ -12,6 +12,7 @@ struct StepVariant {
     int block;             // threads per block
     int cells_per_thread;  // cells handled by one thread
     int vec;               // consecutive cells per lane (1: strided by block)
+    int flags;             // kNtStore | kNtLoad | kXcdRemap
 };
 extern const StepVariant kStepVariants[];
 extern const int kNumStepVariants;

@@ -125,7 +125,7 @@ __device__ inline uint32_t solid3(const uint32_t* __restrict__ bits, uint32_t wo
     return (uint32_t)(v >> ((uint32_t)x & 31u)) & 7u;
 }
 
-template <typename T, int XEDGE, bool FORCE, bool EXPORT>
+template <typename T, int XEDGE, bool FORCE, bool EXPORT, int FLAGS = 0>
 __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T* __restrict__ dst,
                                  const uint32_t* __restrict__ bits, uint32_t words,
                                  float* __restrict__ macro, uint32_t n, int x, int y)
@@ -168,7 +168,7 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
             bool oob = false;
             if (XEDGE == 1 && ex != 0) oob = (sx < 0) || (sx >= nx);
             const uint32_t k = oob ? cc + (uint32_t)inv_of(i) * a.plane : srow + (uint32_t)sx + (uint32_t)i * a.plane;
-            g[i] = Store<T>::load_dev(src, k, w_phys(i), a.inv_scale);
+            g[i] = Store<T>::from_storage(ld<(FLAGS & kNtLoad) != 0>(src + k), a.inv_scale);
         }
         if (XEDGE == 1 && x == 0) {
             const float u0 = a.inlet_ux;
@@ -192,7 +192,8 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
     phys_collide<FORCE>(a, g, rho, ux, uy);
 
 #pragma unroll
-    for (int i = 0; i < 9; ++i) Store<T>::store_dev(dst, c + (uint32_t)i * a.plane, g[i], w_phys(i), a.scale);
+    for (int i = 0; i < 9; ++i)
+        st<(FLAGS & kNtStore) != 0>(dst + c + (uint32_t)i * a.plane, Store<T>::to_storage(g[i], a.scale));
     // Pushes into solid neighbours x + e_i (slot inv(i)); the mask's edge
     // bits are empty for a non-periodic x, so nothing is pushed outside.
     if ((sm | sp | (s0 & 5u)) == 0u) return;
@@ -216,20 +217,20 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
 // One step of the whole lattice.  Block = BS threads on a row segment of
 // BS*CPT cells; thread t takes cells t, t+BS, ... so every load and store
 // instruction of a wavefront covers 64 consecutive cells of one plane.
-template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS, int CPT>
+template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS, int CPT, int FLAGS>
 __global__ __launch_bounds__(BS) void k_phys_step(PhysArgs a, const T* __restrict__ src,
                                                   T* __restrict__ dst, const uint32_t* __restrict__ bits,
                                                   uint32_t words, float* __restrict__ macro, uint32_t nseg)
 {
     const uint32_t total = nseg * (uint32_t)a.ny;
-    const uint32_t tile = xcd_remap(blockIdx.x, total);
+    const uint32_t tile = (FLAGS & kXcdRemap) ? xcd_remap(blockIdx.x, total) : blockIdx.x;
     const int y = (int)(tile / nseg);
     const int x0 = (int)(tile % nseg) * (BS * CPT) + (int)threadIdx.x;
     const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int x = x0 + k * BS;
-        if (x < a.nx) phys_cell<T, XEDGE, FORCE, EXPORT>(a, src, dst, bits, words, macro, n, x, y);
+        if (x < a.nx) phys_cell<T, XEDGE, FORCE, EXPORT, FLAGS>(a, src, dst, bits, words, macro, n, x, y);
     }
 }
 
@@ -369,13 +370,13 @@ __device__ inline void phys_cells4(const PhysArgs& a, const T* __restrict__ src,
     }
 }
 
-template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS>
+template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS, int FLAGS>
 __global__ __launch_bounds__(BS) void k_phys_step4(PhysArgs a, const T* __restrict__ src, T* __restrict__ dst,
                                                    const uint32_t* __restrict__ bits, uint32_t words,
                                                    float* __restrict__ macro, uint32_t nseg)
 {
     const uint32_t total = nseg * (uint32_t)a.ny;
-    const uint32_t tile = xcd_remap(blockIdx.x, total);
+    const uint32_t tile = (FLAGS & kXcdRemap) ? xcd_remap(blockIdx.x, total) : blockIdx.x;
     const int y = (int)(tile / nseg);
     const int x0 = (int)(tile % nseg) * (BS * 4) + 4 * (int)threadIdx.x;
     if (x0 >= a.nx) return;
@@ -384,7 +385,8 @@ __global__ __launch_bounds__(BS) void k_phys_step4(PhysArgs a, const T* __restri
         phys_cells4<T, FORCE, EXPORT>(a, src, dst, bits, words, macro, n, x0, y);
     } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) phys_cell<T, XEDGE, FORCE, EXPORT>(a, src, dst, bits, words, macro, n, x0 + k, y);
+        for (int k = 0; k < 4; ++k)
+            phys_cell<T, XEDGE, FORCE, EXPORT, FLAGS>(a, src, dst, bits, words, macro, n, x0 + k, y);
     }
 }
 
@@ -914,34 +916,45 @@ __global__ __launch_bounds__(256) void k_copy16(const float4* __restrict__ src, 
 
 static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
-template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS, int CPT>
+template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS, int CPT, int FLAGS>
 static hipError_t launch_phys_variant(const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
                                       uint32_t words, float* macro, hipStream_t s)
 {
     const uint32_t nseg = (uint32_t)((a.nx + BS * CPT - 1) / (BS * CPT));
     const uint32_t total = nseg * (uint32_t)a.ny;
-    hipLaunchKernelGGL((k_phys_step<T, XEDGE, FORCE, EXPORT, BS, CPT>), dim3(total), dim3(BS), 0, s, a,
+    hipLaunchKernelGGL((k_phys_step<T, XEDGE, FORCE, EXPORT, BS, CPT, FLAGS>), dim3(total), dim3(BS), 0, s, a,
                        (const T*)src, (T*)dst, bits, words, macro, nseg);
     return hipGetLastError();
 }
 
-template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS>
+template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS, int FLAGS>
 static hipError_t launch_phys_variant4(const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
                                        uint32_t words, float* macro, hipStream_t s)
 {
     if (a.nx % 4 != 0) return hipErrorInvalidValue;
     const uint32_t nseg = (uint32_t)((a.nx + BS * 4 - 1) / (BS * 4));
     const uint32_t total = nseg * (uint32_t)a.ny;
-    hipLaunchKernelGGL((k_phys_step4<T, XEDGE, FORCE, EXPORT, BS>), dim3(total), dim3(BS), 0, s, a,
+    hipLaunchKernelGGL((k_phys_step4<T, XEDGE, FORCE, EXPORT, BS, FLAGS>), dim3(total), dim3(BS), 0, s, a,
                        (const T*)src, (T*)dst, bits, words, macro, nseg);
     return hipGetLastError();
 }
 
-// Tile shapes the step kernel is built for: (block size, cells per thread,
-// consecutive cells per lane).  vec = 4 needs nx % 4 == 0.
+// Tile shapes the step kernel is built for: threads per block, cells per
+// thread, consecutive cells per lane (4 needs nx % 4 == 0) and the kNt* /
+// kXcdRemap flags.  Every shape computes the same bits.
 const StepVariant kStepVariants[] = {
-    {256, 1, 1}, {256, 2, 1}, {256, 4, 1}, {256, 8, 1}, {128, 4, 1}, {512, 2, 1},
-    {512, 4, 1}, {1024, 4, 1}, {64, 4, 4},  {128, 4, 4}, {256, 4, 4}, {512, 4, 4},
+    {256, 1, 1, 0},
+    {256, 1, 1, kNtStore},
+    {256, 1, 1, kNtStore | kNtLoad},
+    {256, 1, 1, kXcdRemap},
+    {256, 2, 1, kNtStore},
+    {128, 1, 1, kNtStore},
+    {512, 1, 1, kNtStore},
+    {256, 4, 1, kNtStore},
+    {256, 1, 1, kNtLoad},
+    {64, 4, 4, kNtStore},
+    {128, 4, 4, 0},
+    {1024, 1, 1, kNtStore},
 };
 const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
 
@@ -955,21 +968,26 @@ template <typename T, int XEDGE, bool FORCE, bool EXPORT>
 static hipError_t dispatch_variant(int v, const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
                                    uint32_t words, float* macro, hipStream_t s)
 {
+    constexpr int S = kNtStore, L = kNtLoad, R = kXcdRemap;
+#define LBM_V(BS, CPT, FL) return launch_phys_variant<T, XEDGE, FORCE, EXPORT, BS, CPT, FL>(a, src, dst, bits, words, macro, s)
+#define LBM_V4(BS, FL) return launch_phys_variant4<T, XEDGE, FORCE, EXPORT, BS, FL>(a, src, dst, bits, words, macro, s)
     switch (v) {
-    case 0: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 1>(a, src, dst, bits, words, macro, s);
-    case 1: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 2>(a, src, dst, bits, words, macro, s);
-    case 2: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 4>(a, src, dst, bits, words, macro, s);
-    case 3: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 256, 8>(a, src, dst, bits, words, macro, s);
-    case 4: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 128, 4>(a, src, dst, bits, words, macro, s);
-    case 5: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 512, 2>(a, src, dst, bits, words, macro, s);
-    case 6: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 512, 4>(a, src, dst, bits, words, macro, s);
-    case 7: return launch_phys_variant<T, XEDGE, FORCE, EXPORT, 1024, 4>(a, src, dst, bits, words, macro, s);
-    case 8: return launch_phys_variant4<T, XEDGE, FORCE, EXPORT, 64>(a, src, dst, bits, words, macro, s);
-    case 9: return launch_phys_variant4<T, XEDGE, FORCE, EXPORT, 128>(a, src, dst, bits, words, macro, s);
-    case 10: return launch_phys_variant4<T, XEDGE, FORCE, EXPORT, 256>(a, src, dst, bits, words, macro, s);
-    case 11: return launch_phys_variant4<T, XEDGE, FORCE, EXPORT, 512>(a, src, dst, bits, words, macro, s);
+    case 0: LBM_V(256, 1, 0);
+    case 1: LBM_V(256, 1, S);
+    case 2: LBM_V(256, 1, S | L);
+    case 3: LBM_V(256, 1, R);
+    case 4: LBM_V(256, 2, S);
+    case 5: LBM_V(128, 1, S);
+    case 6: LBM_V(512, 1, S);
+    case 7: LBM_V(256, 4, S);
+    case 8: LBM_V(256, 1, L);
+    case 9: LBM_V4(64, S);
+    case 10: LBM_V4(128, 0);
+    case 11: LBM_V(1024, 1, S);
     default: return hipErrorInvalidValue;
     }
+#undef LBM_V
+#undef LBM_V4
 }
 
 template <typename T, int XEDGE>

@@ -85,7 +85,7 @@ _SIGNATURES = [
     ("lbm_macro", C.c_int, [_P, _P, _P, _P]),
     ("lbm_reduce", C.c_int, [_P, _P]),
     ("lbm_variant_count", C.c_int, []),
-    ("lbm_variant_info", C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("lbm_variant_info", C.c_int, [C.c_int] + [C.POINTER(C.c_int)] * 4),
     ("lbm_set_variant", C.c_int, [_P, C.c_int]),
     ("lbm_get_variant", C.c_int, [_P, C.POINTER(C.c_int)]),
     ("lbm_autotune", C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
@@ -164,14 +164,15 @@ def device_count() -> int:
     return n.value
 
 
-def variant_shapes() -> list[tuple[int, int, int]]:
-    """(threads per block, cells per thread, consecutive cells per lane)."""
+def variant_shapes() -> list[tuple[int, int, int, int]]:
+    """(threads per block, cells per thread, consecutive cells per lane,
+    flags: 1 non-temporal stores, 2 non-temporal loads, 4 rows per XCD)."""
     lib = load_library()
     out = []
     for v in range(lib.lbm_variant_count()):
-        b, c, w = C.c_int(), C.c_int(), C.c_int()
-        _check(lib.lbm_variant_info(v, C.byref(b), C.byref(c), C.byref(w)))
-        out.append((b.value, c.value, w.value))
+        b, c, w, f = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(lib.lbm_variant_info(v, C.byref(b), C.byref(c), C.byref(w), C.byref(f)))
+        out.append((b.value, c.value, w.value, f.value))
     return out
 
 
