@@ -182,7 +182,8 @@ def main():
     t = time.perf_counter()
     scene = W.periodic_disks(n, n)
     log(f"scene {n}x{n}: {len(scene.disks)} disks, solid {scene.solid_fraction:.4f} ({time.perf_counter()-t:.1f} s)")
-    lat = L.Lattice(n, n, tau, mode=L.MODE_PHYSICS, force=(g, 0.0))
+    storage = L.STORAGE_F16_DEV if args.f16 else L.STORAGE_F32
+    lat = L.Lattice(n, n, tau, mode=L.MODE_PHYSICS, storage=storage, force=(g, 0.0))
     lat.write_materials(W.materials_from_solid(scene.solid))
     lat.reset()
     lat.set_graph_chunk(args.graph_chunk)
@@ -204,8 +205,9 @@ def main():
     sec = ms / 1e3
     mlups = cells * K / sec / 1e6
     step_us = ms * 1e3 / K
-    achieved = BYTES_PER_CELL_F32 * cells / (step_us * 1e-6) / 1e9
-    workload = f"d2q9_phys_f32_{n}x{n}"
+    bytes_per_cell = 36 if args.f16 else BYTES_PER_CELL_F32
+    achieved = bytes_per_cell * cells / (step_us * 1e-6) / 1e9
+    workload = f"d2q9_phys_{'f16dev' if args.f16 else 'f32'}_{n}x{n}"
     r = lat.reduce()
     log(f"{K} steps: {ms:.1f} ms events, {wall*1e3:.1f} ms wall, {step_us:.2f} us/step, {mlups:.0f} MLUPS, "
         f"{achieved:.0f} GB/s; mass {r[0]:.9e} drag ({r[3]:.3e}, {r[4]:.3e})")
@@ -229,7 +231,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f16-deviation storage, f32 math" if args.f16 else "f32",
         "data": "synthetic",
         "config": {
             "workload": workload,
