@@ -278,6 +278,65 @@ int lbm_batch_get_variant(lbm_batch* b, int* variant);
 int lbm_batch_set_graph_chunk(lbm_batch* b, int chunk);
 int lbm_batch_autotune(lbm_batch* b, int steps, int* best_variant, double* ms_per_step);
 
+/* ---- learned collision correction --------------------------------------
+ * No reference counterpart.  A per-cell network 9 -> 64 -> 64 -> 9 (GELU,
+ * erf form) maps the non-equilibrium part of the gathered populations,
+ * x_i = (f_i - feq_i) / (w_i * in_scale), to a correction
+ * df_i = w_i * out_scale * y_i; the correction is projected onto the
+ * non-conserved moments (its mass and momentum are removed) and added to the
+ * BGK post-collision populations.  Parameters: one vector of 5385 floats,
+ * W1[64][9] b1[64] W2[64][64] b2[64] W3[9][64] b3[9] (row = output unit).
+ * Training (forward, backward, Adam) runs on the device in binary32. */
+typedef struct lbm_mlp lbm_mlp;
+
+int lbm_mlp_param_count(void);
+/* Glorot-uniform weights from SplitMix64(seed), zero biases; room for
+ * `capacity` training samples on the device. */
+int lbm_mlp_create(uint64_t seed, int capacity, lbm_mlp** out);
+int lbm_mlp_destroy(lbm_mlp* m);
+int lbm_mlp_set_params(lbm_mlp* m, const float* params);
+int lbm_mlp_get_params(lbm_mlp* m, float* params);
+int lbm_mlp_set_scales(lbm_mlp* m, float in_scale, float out_scale);
+int lbm_mlp_get_scales(lbm_mlp* m, float* in_scale, float* out_scale);
+/* The raw network on n samples (x: n*9 floats, y: n*9 floats, host). */
+int lbm_mlp_forward(lbm_mlp* m, const float* x, int n, float* y);
+/* Loss 1/(9n) sum (y - t)^2 of n host samples and its gradient (5385). */
+int lbm_mlp_loss_grad(lbm_mlp* m, const float* x, const float* t, int n, float* grad, double* loss);
+/* Training samples held on the device. */
+int lbm_mlp_clear_samples(lbm_mlp* m);
+int lbm_mlp_sample_count(lbm_mlp* m, int* n);
+int lbm_mlp_add_samples(lbm_mlp* m, const float* x, const float* t, int n);
+/* Samples from two consecutive pre-collision states of a periodic lattice
+ * (dense f[i*nx*ny + c], binary32): for every stride-th cell from offset
+ * whose 3x3 neighbourhood is fluid, x = the network input at time t and
+ * t = the projected residual f_i(c + e_i, t+1) - BGK_i(f(c, t)) in output
+ * units.  Uses the current scales. */
+int lbm_mlp_add_transition(lbm_mlp* m, const float* state_t, const float* state_t1, const uint8_t* solid, int nx,
+                           int ny, float tau, float force_x, float force_y, int stride, int offset, int* added);
+/* Adam (beta1 0.9, beta2 0.999, eps 1e-8) over `steps` minibatches of
+ * `batch` samples drawn from the stored ones with SplitMix64(seed). */
+int lbm_mlp_train(lbm_mlp* m, int steps, int batch, float lr, uint64_t seed, double* last_loss);
+
+/* Physics-mode handles: steps add the network's projected correction to
+ * the BGK collision (binary32 network, parameters staged in LDS); NULL
+ * switches back to plain BGK.  The network must outlive its use. */
+int lbm_set_learned_collision(lbm_handle* h, const lbm_mlp* m);
+/* Largest |u| over fluid cells of the current state and whether every
+ * fluid cell is finite with rho > 0. */
+int lbm_health(lbm_handle* h, float* umax, int* finite);
+/* nsteps steps in chunks of `chunk` (even): after each chunk the health is
+ * checked; a chunk run with the learned correction that ends non-finite or
+ * with |u| > umax_limit is rerun from its first state with plain BGK (the
+ * fallback), and *fallbacks counts those chunks. */
+int lbm_run_guarded(lbm_handle* h, int nsteps, int chunk, float umax_limit, int* fallbacks);
+/* 4x restriction of the current physics-mode state to an (nx/4) x (ny/4)
+ * lattice for tau_coarse and the coarse body force: dense pre-collision
+ * coarse populations (9*nc floats), the coarse solid map (a coarse cell is
+ * solid when >= 8 of its 16 fine cells are) and, optionally, the coarse
+ * velocity (mass-weighted block mean of the fine velocity). */
+int lbm_restrict4(lbm_handle* h, float tau_coarse, float force_x_coarse, float force_y_coarse, float* coarse_f,
+                  uint8_t* coarse_solid, float* coarse_ux, float* coarse_uy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,7 +17,7 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "_lib"
 LIB = LIBDIR / "libsimuverse_lbm.so"
 
-SOURCES = ["lbm_kernels.hip", "lbm_api.hip", "lbm_batch_api.hip"]
+SOURCES = ["lbm_kernels.hip", "lbm_api.hip", "lbm_batch_api.hip", "lbm_mlp.hip"]
 
 # -ffp-contract=off: the parity kernel must round exactly where the
 # reference's WGSL rounds, and the physics kernels stay reproducible

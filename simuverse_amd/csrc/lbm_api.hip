@@ -6,6 +6,7 @@
 // HIP graphs of 100-step chunks.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -81,10 +82,13 @@ struct lbm_handle {
     int cur = 0;
     float omega = 1.0f;
     float scale = 1.0f, inv_scale = 1.0f;
-    int variant = 2;
+    int variant = 6;
     int graph_chunk = 100;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    const lbm_mlp* mlp = nullptr;   // learned collision correction, if set
+    uint32_t* health = nullptr;     // [max |u|^2 bits, non-finite flag]
+    void* backup = nullptr;         // chunk start state of lbm_run_guarded
 
     bool physics() const { return d.mode == LBM_MODE_PHYSICS; }
     bool force() const { return d.force_x != 0.0f || d.force_y != 0.0f; }
@@ -137,6 +141,13 @@ hipError_t enqueue_step(lbm_handle* h, int swap, bool export_macro)
 {
     const void* src = h->buf[swap];
     void* dst = h->buf[swap ^ 1];
+    if (h->physics() && h->mlp) {
+        if (export_macro) return hipErrorNotSupported;
+        float s_in, s_out;
+        lbm_mlp_scales(h->mlp, &s_in, &s_out);
+        return launch_phys_step_learned(h->f16, h->d.x_edge, h->force(), h->phys(), src, dst, h->bits, h->words,
+                                        lbm_mlp_device_params(h->mlp), s_in, s_out, h->stream);
+    }
     if (h->physics())
         return launch_phys_step(h->variant, h->f16, h->d.x_edge, h->force(), export_macro, h->phys(), src, dst,
                                 h->bits, h->words, h->macro, h->stream);
@@ -265,7 +276,8 @@ void free_handle(lbm_handle* h)
 {
     drop_graphs(h);
     for (void* p : {h->buf[0], h->buf[1], (void*)h->mat, (void*)h->biter, (void*)h->fvel, (void*)h->macro,
-                    (void*)h->tmp3, (void*)h->partial, (void*)h->red, (void*)h->bits})
+                    (void*)h->tmp3, (void*)h->partial, (void*)h->red, (void*)h->bits, (void*)h->health,
+                    h->backup})
         if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -735,6 +747,108 @@ int lbm_load_state(lbm_handle* h, const void* in, size_t bytes)
         return fail(LBM_ERR_INVALID, "state buffer must be %zu bytes", h->pop_bytes());
     HIP_TRY(hipMemcpyAsync(h->buf[h->cur], in, bytes, hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    return LBM_OK;
+}
+
+// ---- learned collision ------------------------------------------------------
+
+int lbm_set_learned_collision(lbm_handle* h, const lbm_mlp* m)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (m && !h->physics()) return lbm_fail(LBM_ERR_UNSUPPORTED, "the learned correction needs physics mode");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->mlp = m;
+    drop_graphs(h);
+    return LBM_OK;
+}
+
+int lbm_health(lbm_handle* h, float* umax, int* finite)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!h->physics()) return lbm_fail(LBM_ERR_UNSUPPORTED, "health check needs physics mode");
+    if (!h->health) HIP_TRY(hipMalloc(&h->health, 2 * sizeof(uint32_t)));
+    HIP_TRY(hipMemsetAsync(h->health, 0, 2 * sizeof(uint32_t), h->stream));
+    HIP_TRY(launch_phys_health(h->f16, h->d.x_edge, h->force(), h->phys(), h->buf[h->cur], h->mat, h->health,
+                               h->stream));
+    uint32_t out[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(out, h->health, sizeof out, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    float u2;
+    std::memcpy(&u2, &out[0], sizeof u2);
+    if (umax) *umax = std::sqrt(u2);
+    if (finite) *finite = out[1] ? 0 : 1;
+    return LBM_OK;
+}
+
+int lbm_run_guarded(lbm_handle* h, int nsteps, int chunk, float umax_limit, int* fallbacks)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (nsteps < 0 || chunk < 2 || (chunk & 1) || !(umax_limit > 0.0f))
+        return lbm_fail(LBM_ERR_INVALID, "bad arguments (chunk must be even and >= 2)");
+    if (!h->physics()) return lbm_fail(LBM_ERR_UNSUPPORTED, "guarded runs need physics mode");
+    if (!h->backup) HIP_TRY(hipMalloc(&h->backup, h->pop_bytes()));
+    int falls = 0;
+    for (int done = 0; done < nsteps;) {
+        const int len = std::min(chunk, nsteps - done);
+        const int cur0 = h->cur;
+        HIP_TRY(hipMemcpyAsync(h->backup, h->buf[cur0], h->pop_bytes(), hipMemcpyDeviceToDevice, h->stream));
+        if ((rc = enqueue_run(h, len, false))) return rc;
+        float umax = 0.0f;
+        int finite = 0;
+        if ((rc = lbm_health(h, &umax, &finite))) return rc;
+        if (h->mlp && (!finite || !(umax <= umax_limit))) {
+            // The chunk with the learned correction left the healthy range:
+            // restart it from its first state with plain BGK.
+            HIP_TRY(hipMemcpyAsync(h->buf[cur0], h->backup, h->pop_bytes(), hipMemcpyDeviceToDevice, h->stream));
+            h->cur = cur0;
+            const lbm_mlp* keep = h->mlp;
+            h->mlp = nullptr;
+            drop_graphs(h);
+            rc = enqueue_run(h, len, false);
+            h->mlp = keep;
+            drop_graphs(h);
+            if (rc) return rc;
+            ++falls;
+        }
+        done += len;
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (fallbacks) *fallbacks = falls;
+    return LBM_OK;
+}
+
+int lbm_restrict4(lbm_handle* h, float tau_coarse, float force_x_coarse, float force_y_coarse, float* coarse_f,
+                  uint8_t* coarse_solid, float* coarse_ux, float* coarse_uy)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!h->physics()) return lbm_fail(LBM_ERR_UNSUPPORTED, "restriction needs physics mode");
+    if (h->d.nx % 4 || h->d.ny % 4) return lbm_fail(LBM_ERR_INVALID, "restriction needs nx and ny divisible by 4");
+    if (!coarse_f || !coarse_solid || !(tau_coarse > 0.5f)) return lbm_fail(LBM_ERR_INVALID, "bad arguments");
+    const size_t nc = (size_t)(h->d.nx / 4) * (size_t)(h->d.ny / 4);
+    float* df = nullptr;
+    uint8_t* ds = nullptr;
+    float* du = nullptr;
+    HIP_TRY(hipMalloc(&df, 11 * nc * sizeof(float)));
+    du = df + 9 * nc;
+    hipError_t e = hipMalloc(&ds, nc);
+    const float nscale = 4.0f * tau_coarse / h->d.tau;
+    if (e == hipSuccess)
+        e = launch_restrict4(h->f16, h->d.x_edge, h->force(), h->phys(), h->buf[h->cur], h->mat, nscale,
+                             force_x_coarse, force_y_coarse, df, ds, du, du + nc, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(coarse_f, df, 9 * nc * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(coarse_solid, ds, nc, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && coarse_ux)
+        e = hipMemcpyAsync(coarse_ux, du, nc * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && coarse_uy)
+        e = hipMemcpyAsync(coarse_uy, du + nc, nc * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(df);
+    if (ds) (void)hipFree(ds);
+    if (e != hipSuccess) return lbm_fail(LBM_ERR_HIP, "restriction: %s", hipGetErrorString(e));
     return LBM_OK;
 }
 

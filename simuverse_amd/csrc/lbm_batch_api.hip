@@ -54,7 +54,7 @@ struct lbm_batch {
     double* partial = nullptr;
     double* red = nullptr;
     int cur = 0;
-    int variant = 3;
+    int variant = 0;
     int cus = 256;
     int graph_chunk = 100;
     hipGraphExec_t graph[2] = {nullptr, nullptr};

@@ -151,6 +151,65 @@ struct WgslArgs {
     float omega;
 };
 
+// The learned collision correction's network 9 -> 64 -> 64 -> 9: offsets of
+// W1[64][9], b1[64], W2[64][64], b2[64], W3[9][64], b3[9] in its flat
+// parameter vector.
+constexpr int kMlpIn = 9, kMlpH = 64, kMlpOut = 9;
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kMlpH * kMlpIn;
+constexpr int kOffW2 = kOffB1 + kMlpH;
+constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH;
+constexpr int kOffW3 = kOffB2 + kMlpH;
+constexpr int kOffB3 = kOffW3 + kMlpOut * kMlpH;
+constexpr int kMlpParams = kOffB3 + kMlpOut;  // 5385
+
+__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ inline float gelu_grad(float x)
+{
+    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * expf(-0.5f * x * x) * 0.39894228040143268f;
+}
+
+// The network on one sample in binary32; P holds the parameter vector
+// (in LDS in the kernels that call this).
+__device__ inline void mlp_forward_one(const float* __restrict__ P, const float x[kMlpIn], float y[kMlpOut])
+{
+    float h1[kMlpH];
+#pragma unroll 4
+    for (int j = 0; j < kMlpH; ++j) {
+        float s = P[kOffB1 + j];
+#pragma unroll
+        for (int i = 0; i < kMlpIn; ++i) s += P[kOffW1 + j * kMlpIn + i] * x[i];
+        h1[j] = gelu(s);
+    }
+#pragma unroll
+    for (int m = 0; m < kMlpOut; ++m) y[m] = P[kOffB3 + m];
+    for (int j = 0; j < kMlpH; ++j) {
+        float s = P[kOffB2 + j];
+#pragma unroll
+        for (int k = 0; k < kMlpH; ++k) s += P[kOffW2 + j * kMlpH + k] * h1[k];
+        const float h2 = gelu(s);
+#pragma unroll
+        for (int m = 0; m < kMlpOut; ++m) y[m] += P[kOffW3 + m * kMlpH + j] * h2;
+    }
+}
+
+// Projection of a collision correction onto the non-conserved moments:
+// d_i - w_i (a + 3 e_i.b) with a = sum d, b = sum e d (exact weights give
+// sum w = 1, sum w e = 0, sum w e e = I/3), so mass and momentum of the
+// result vanish.
+__device__ inline void project_correction(float d[9])
+{
+    float a = 0.0f, bx = 0.0f, by = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        a += d[i];
+        bx += (float)ex_of(i) * d[i];
+        by += (float)ey_of(i) * d[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) d[i] -= w_phys(i) * (a + 3.0f * ((float)ex_of(i) * bx + (float)ey_of(i) * by));
+}
+
 // Linear block id -> logical tile id such that the blocks the hardware
 // dispatches to one XCD (id mod 8) get a contiguous range of tiles.
 __device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t total)

@@ -55,12 +55,32 @@ struct BatchVariant {
     int block;           // threads per block
     int vec;             // consecutive cells per lane
     int blocks_per_cu;   // persistent grid = blocks_per_cu * CUs
+    int flags;           // kNtStore | kXcdRemap
 };
 extern const BatchVariant kBatchVariants[];
 extern const int kNumBatchVariants;
 hipError_t launch_batch_step(int variant, bool f16, const BatchDomainDev* doms, const uint16_t* chunk_dom,
                              uint32_t nchunks, uint32_t grid, const void* src, void* dst, const uint32_t* bits,
                              hipStream_t s);
+
+// Learned collision, restriction and health check.
+hipError_t launch_phys_step_learned(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, void* dst,
+                                   const uint32_t* bits, uint32_t words, const float* params, float s_in,
+                                   float s_out, hipStream_t s);
+hipError_t launch_restrict4(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, const uint8_t* mat,
+                            float nscale, float gcx, float gcy, float* fc, uint8_t* solid_c, float* uxc, float* uyc,
+                            hipStream_t s);
+hipError_t launch_phys_health(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, const uint8_t* mat,
+                              uint32_t* out, hipStream_t s);
+
+}  // namespace lbm
+
+struct lbm_mlp;
+// Device pointer to a network's binary32 parameters, and its scales.
+const float* lbm_mlp_device_params(const lbm_mlp* m);
+void lbm_mlp_scales(const lbm_mlp* m, float* s_in, float* s_out);
+
+namespace lbm {
 
 // Number of blocks of the binary64 reduction (partials = 5 doubles each).
 constexpr int kReduceBlocks = 1024;

@@ -125,10 +125,13 @@ __device__ inline uint32_t solid3(const uint32_t* __restrict__ bits, uint32_t wo
     return (uint32_t)(v >> ((uint32_t)x & 31u)) & 7u;
 }
 
-template <typename T, int XEDGE, bool FORCE, bool EXPORT, int FLAGS = 0>
+// LEARNED adds the network's projected correction to the collision; mlp
+// then points at its parameters (in LDS) and s_in, s_out are its scales.
+template <typename T, int XEDGE, bool FORCE, bool EXPORT, int FLAGS = 0, bool LEARNED = false>
 __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T* __restrict__ dst,
                                  const uint32_t* __restrict__ bits, uint32_t words,
-                                 float* __restrict__ macro, uint32_t n, int x, int y)
+                                 float* __restrict__ macro, uint32_t n, int x, int y,
+                                 const float* __restrict__ mlp = nullptr, float s_in = 1.0f, float s_out = 1.0f)
 {
     const int nx = a.nx, ny = a.ny;
     const uint32_t r0 = (uint32_t)y * (uint32_t)nx;
@@ -189,7 +192,27 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
         macro[c + 2 * n] = rho;
         macro[c + 3 * n] = 1.0f;
     }
+    float corr[9];
+    if constexpr (LEARNED) {
+        // network input: non-equilibrium part (g - geq = f - feq) over w s_in
+        const float dr = rho - 1.0f, usq = 1.5f * (ux * ux + uy * uy);
+        float xin[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const float eu = (float)ex_of(i) * ux + (float)ey_of(i) * uy;
+            const float geq = w_phys(i) * (dr + rho * (3.0f * eu + 4.5f * (eu * eu) - usq));
+            xin[i] = (g[i] - geq) / (w_phys(i) * s_in);
+        }
+        mlp_forward_one(mlp, xin, corr);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) corr[i] *= w_phys(i) * s_out;
+        project_correction(corr);
+    }
     phys_collide<FORCE>(a, g, rho, ux, uy);
+    if constexpr (LEARNED) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) g[i] += corr[i];
+    }
 
 #pragma unroll
     for (int i = 0; i < 9; ++i)
@@ -395,7 +418,7 @@ __global__ __launch_bounds__(BS) void k_phys_step4(PhysArgs a, const T* __restri
 // ---------------------------------------------------------------------------
 
 // The cells of chunk t (BS*V consecutive cells of one domain, row major).
-template <typename T, int XEDGE, bool FORCE, int BS, int V>
+template <typename T, int XEDGE, bool FORCE, int BS, int V, int FLAGS>
 __device__ inline void batch_chunk(const BatchDomainDev& D, uint32_t t, const T* __restrict__ src,
                                    T* __restrict__ dst, const uint32_t* __restrict__ bits)
 {
@@ -412,33 +435,151 @@ __device__ inline void batch_chunk(const BatchDomainDev& D, uint32_t t, const T*
         } else {
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                phys_cell<T, XEDGE, FORCE, false>(D.a, s, o, m, D.words, nullptr, D.ncells, x0 + k, y);
+                phys_cell<T, XEDGE, FORCE, false, FLAGS>(D.a, s, o, m, D.words, nullptr, D.ncells, x0 + k, y);
         }
     } else {
-        phys_cell<T, XEDGE, FORCE, false>(D.a, s, o, m, D.words, nullptr, D.ncells, x0, y);
+        phys_cell<T, XEDGE, FORCE, false, FLAGS>(D.a, s, o, m, D.words, nullptr, D.ncells, x0, y);
     }
 }
 
 // Block b (after the XCD remap) steps chunks [b*per_block, (b+1)*per_block).
-template <typename T, int BS, int V>
+template <typename T, int BS, int V, int FLAGS>
 __global__ __launch_bounds__(BS) void k_batch_step(const BatchDomainDev* __restrict__ doms,
                                                    const uint16_t* __restrict__ chunk_dom, uint32_t nchunks,
                                                    uint32_t per_block, const T* __restrict__ src, T* __restrict__ dst,
                                                    const uint32_t* __restrict__ bits)
 {
-    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t b = (FLAGS & kXcdRemap) ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t t0 = b * per_block;
     const uint32_t t1 = min(t0 + per_block, nchunks);
     for (uint32_t t = t0; t < t1; ++t) {
         const BatchDomainDev D = doms[chunk_dom[t]];
         if (D.x_edge) {
-            if (D.force) batch_chunk<T, 1, true, BS, V>(D, t, src, dst, bits);
-            else batch_chunk<T, 1, false, BS, V>(D, t, src, dst, bits);
+            if (D.force) batch_chunk<T, 1, true, BS, V, FLAGS>(D, t, src, dst, bits);
+            else batch_chunk<T, 1, false, BS, V, FLAGS>(D, t, src, dst, bits);
         } else {
-            if (D.force) batch_chunk<T, 0, true, BS, V>(D, t, src, dst, bits);
-            else batch_chunk<T, 0, false, BS, V>(D, t, src, dst, bits);
+            if (D.force) batch_chunk<T, 0, true, BS, V, FLAGS>(D, t, src, dst, bits);
+            else batch_chunk<T, 0, false, BS, V, FLAGS>(D, t, src, dst, bits);
         }
     }
+}
+
+// 4x restriction of a fine physics-mode state to a coarse lattice
+// (nx/4 x ny/4), one coarse cell per thread.  A coarse cell is solid when at
+// least 8 of its 16 fine cells are; otherwise, over its fluid fine cells,
+//   rho_c = mean rho,  u_c = sum(rho u) / sum(rho),
+//   f_c = feq(rho_c, u_c) + P(4 tau_c/tau_f * mean(f - feq)) - 3 w e.(g_c/2)
+// with P the projection that removes mass and momentum: the non-equilibrium
+// part is rescaled for a 4x coarser lattice under acoustic scaling
+// (Dupuis & Chopard, Phys. Rev. E 67 (2003) 066707), and the last term makes
+// the coarse Guo velocity (sum e f + g_c/2)/rho equal u_c.  Output: dense
+// pre-collision f (binary32), the solid map and u_c.
+template <typename T, int XEDGE, bool FORCE>
+__global__ __launch_bounds__(256) void k_restrict4(PhysArgs a, const T* __restrict__ src,
+                                                   const uint8_t* __restrict__ mat, float nscale, float gcx, float gcy,
+                                                   float* __restrict__ fc, uint8_t* __restrict__ solid_c,
+                                                   float* __restrict__ uxc, float* __restrict__ uyc)
+{
+    const int cx_n = a.nx / 4, cy_n = a.ny / 4;
+    const uint32_t nc = (uint32_t)cx_n * (uint32_t)cy_n;
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= nc) return;
+    const int X = (int)(k % (uint32_t)cx_n), Y = (int)(k / (uint32_t)cx_n);
+    float sr = 0.0f, sjx = 0.0f, sjy = 0.0f, sneq[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) sneq[i] = 0.0f;
+    int nfl = 0;
+    for (int j = 0; j < 4; ++j)
+        for (int i4 = 0; i4 < 4; ++i4) {
+            const int x = 4 * X + i4, y = 4 * Y + j;
+            if (mat_solid(mat[(uint32_t)y * (uint32_t)a.nx + (uint32_t)x])) continue;
+            ++nfl;
+            float g[9], rho, ux, uy;
+            phys_gather<T, XEDGE>(a, src, mat, x, y, g);
+            phys_moments<FORCE>(a, g, rho, ux, uy);
+            sr += rho;
+            sjx += rho * ux;
+            sjy += rho * uy;
+            const float dr = rho - 1.0f, usq = 1.5f * (ux * ux + uy * uy);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                const float eu = (float)ex_of(i) * ux + (float)ey_of(i) * uy;
+                sneq[i] += g[i] - w_phys(i) * (dr + rho * (3.0f * eu + 4.5f * (eu * eu) - usq));
+            }
+        }
+    const uint32_t n = nc;
+    if (nfl < 8) {
+        solid_c[k] = 1;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) fc[(size_t)i * n + k] = 0.0f;
+        if (uxc) uxc[k] = 0.0f;
+        if (uyc) uyc[k] = 0.0f;
+        return;
+    }
+    solid_c[k] = 0;
+    const float rho = sr / (float)nfl, ux = sjx / sr, uy = sjy / sr;
+    float neq[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) neq[i] = nscale * sneq[i] / (float)nfl;
+    project_correction(neq);
+    const float usq = 1.5f * (ux * ux + uy * uy);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float exf = (float)ex_of(i), eyf = (float)ey_of(i);
+        const float eu = exf * ux + eyf * uy;
+        const float feq = w_phys(i) * rho * (1.0f + 3.0f * eu + 4.5f * (eu * eu) - usq);
+        fc[(size_t)i * n + k] = feq + neq[i] - 1.5f * w_phys(i) * (exf * gcx + eyf * gcy);
+    }
+    if (uxc) uxc[k] = ux;
+    if (uyc) uyc[k] = uy;
+}
+
+// Health of a state: largest |u|^2 over fluid cells (as the bits of a
+// non-negative float, so unsigned max orders them) and whether any cell is
+// not finite.
+template <typename T, int XEDGE, bool FORCE>
+__global__ __launch_bounds__(256) void k_phys_health(PhysArgs a, const T* __restrict__ src,
+                                                     const uint8_t* __restrict__ mat, uint32_t* __restrict__ out)
+{
+    const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    uint32_t u2bits = 0, bad = 0;
+    if (c < n && !mat_solid(mat[c])) {
+        const int y = (int)(c / (uint32_t)a.nx), x = (int)(c % (uint32_t)a.nx);
+        float g[9], rho, ux, uy;
+        phys_gather<T, XEDGE>(a, src, mat, x, y, g);
+        phys_moments<FORCE>(a, g, rho, ux, uy);
+        const float u2 = ux * ux + uy * uy;
+        if (!(u2 <= 3.0e38f) || !(rho > 0.0f)) bad = 1;
+        else u2bits = __float_as_uint(u2);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        u2bits = max(u2bits, (uint32_t)__shfl_xor((int)u2bits, off, 64));
+        bad |= (uint32_t)__shfl_xor((int)bad, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(out, u2bits);
+        if (bad) atomicOr(out + 1, 1u);
+    }
+}
+
+// One step with the learned collision correction (binary32 network, its
+// parameters staged in LDS once per block).  One cell per thread.
+template <typename T, int XEDGE, bool FORCE>
+__global__ __launch_bounds__(256) void k_phys_step_learned(PhysArgs a, const T* __restrict__ src,
+                                                           T* __restrict__ dst, const uint32_t* __restrict__ bits,
+                                                           uint32_t words, const float* __restrict__ params,
+                                                           float s_in, float s_out, uint32_t nseg)
+{
+    __shared__ float P[kMlpParams];
+    for (int k = threadIdx.x; k < kMlpParams; k += 256) P[k] = params[k];
+    __syncthreads();
+    const int y = (int)(blockIdx.x / nseg);
+    const int x = (int)(blockIdx.x % nseg) * 256 + (int)threadIdx.x;
+    if (x >= a.nx) return;
+    phys_cell<T, XEDGE, FORCE, false, kNtStore, true>(a, src, dst, bits, words, nullptr,
+                                                      (uint32_t)a.nx * (uint32_t)a.ny, x, y, P, s_in, s_out);
 }
 
 // Builds the packed solid mask from the material bytes (one 32-bit word per
@@ -1019,7 +1160,8 @@ hipError_t launch_phys_step(int variant, bool f16, int xedge, bool force, bool e
 // Batched step shapes: (threads per block, consecutive cells per lane,
 // persistent blocks per CU).  A chunk is block*vec cells.
 const BatchVariant kBatchVariants[] = {
-    {256, 1, 8}, {64, 4, 16}, {128, 4, 8}, {256, 4, 4}, {256, 4, 8}, {128, 4, 16},
+    {256, 1, 8, kNtStore}, {256, 1, 8, kNtStore | kXcdRemap}, {512, 1, 4, kNtStore}, {256, 1, 8, 0},
+    {64, 4, 16, kNtStore}, {128, 4, 8, 0},
 };
 const int kNumBatchVariants = (int)(sizeof(kBatchVariants) / sizeof(kBatchVariants[0]));
 
@@ -1028,16 +1170,16 @@ static hipError_t launch_batch_typed(int v, const BatchDomainDev* doms, const ui
                                      uint32_t grid, const void* src, void* dst, const uint32_t* bits, hipStream_t s)
 {
     const uint32_t per_block = (nchunks + grid - 1) / grid;
-#define LBM_BATCH(BS, V)                                                                                   \
-    hipLaunchKernelGGL((k_batch_step<T, BS, V>), dim3(grid), dim3(BS), 0, s, doms, chunk_dom, nchunks,  \
+#define LBM_BATCH(BS, V, FL)                                                                                   \
+    hipLaunchKernelGGL((k_batch_step<T, BS, V, FL>), dim3(grid), dim3(BS), 0, s, doms, chunk_dom, nchunks,  \
                        per_block, (const T*)src, (T*)dst, bits)
     switch (v) {
-    case 0: LBM_BATCH(256, 1); break;
-    case 1: LBM_BATCH(64, 4); break;
-    case 2: LBM_BATCH(128, 4); break;
-    case 3: LBM_BATCH(256, 4); break;
-    case 4: LBM_BATCH(256, 4); break;
-    case 5: LBM_BATCH(128, 4); break;
+    case 0: LBM_BATCH(256, 1, kNtStore); break;
+    case 1: LBM_BATCH(256, 1, kNtStore | kXcdRemap); break;
+    case 2: LBM_BATCH(512, 1, kNtStore); break;
+    case 3: LBM_BATCH(256, 1, 0); break;
+    case 4: LBM_BATCH(64, 4, kNtStore); break;
+    case 5: LBM_BATCH(128, 4, 0); break;
     default: return hipErrorInvalidValue;
     }
 #undef LBM_BATCH
@@ -1051,6 +1193,62 @@ hipError_t launch_batch_step(int variant, bool f16, const BatchDomainDev* doms, 
     if (nchunks == 0) return hipSuccess;
     return f16 ? launch_batch_typed<_Float16>(variant, doms, chunk_dom, nchunks, grid, src, dst, bits, s)
                : launch_batch_typed<float>(variant, doms, chunk_dom, nchunks, grid, src, dst, bits, s);
+}
+
+hipError_t launch_restrict4(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, const uint8_t* mat,
+                            float nscale, float gcx, float gcy, float* fc, uint8_t* solid_c, float* uxc, float* uyc,
+                            hipStream_t s)
+{
+    const unsigned nb = blocks_for((size_t)(a.nx / 4) * (size_t)(a.ny / 4), 256);
+#define LBM_R(T, XE, FO)                                                                                       \
+    hipLaunchKernelGGL((k_restrict4<T, XE, FO>), dim3(nb), dim3(256), 0, s, a, (const T*)src, mat, nscale, gcx, \
+                       gcy, fc, solid_c, uxc, uyc)
+    if (f16) {
+        if (xedge) { if (force) LBM_R(_Float16, 1, true); else LBM_R(_Float16, 1, false); }
+        else { if (force) LBM_R(_Float16, 0, true); else LBM_R(_Float16, 0, false); }
+    } else {
+        if (xedge) { if (force) LBM_R(float, 1, true); else LBM_R(float, 1, false); }
+        else { if (force) LBM_R(float, 0, true); else LBM_R(float, 0, false); }
+    }
+#undef LBM_R
+    return hipGetLastError();
+}
+
+hipError_t launch_phys_health(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, const uint8_t* mat,
+                              uint32_t* out, hipStream_t s)
+{
+    const unsigned nb = blocks_for((size_t)a.nx * a.ny, 256);
+#define LBM_H(T, XE, FO) \
+    hipLaunchKernelGGL((k_phys_health<T, XE, FO>), dim3(nb), dim3(256), 0, s, a, (const T*)src, mat, out)
+    if (f16) {
+        if (xedge) { if (force) LBM_H(_Float16, 1, true); else LBM_H(_Float16, 1, false); }
+        else { if (force) LBM_H(_Float16, 0, true); else LBM_H(_Float16, 0, false); }
+    } else {
+        if (xedge) { if (force) LBM_H(float, 1, true); else LBM_H(float, 1, false); }
+        else { if (force) LBM_H(float, 0, true); else LBM_H(float, 0, false); }
+    }
+#undef LBM_H
+    return hipGetLastError();
+}
+
+hipError_t launch_phys_step_learned(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, void* dst,
+                                   const uint32_t* bits, uint32_t words, const float* params, float s_in,
+                                   float s_out, hipStream_t s)
+{
+    const uint32_t nseg = (uint32_t)((a.nx + 255) / 256);
+    const dim3 grid(nseg * (uint32_t)a.ny), block(256);
+#define LBM_L(T, XE, FO)                                                                                  \
+    hipLaunchKernelGGL((k_phys_step_learned<T, XE, FO>), grid, block, 0, s, a, (const T*)src, (T*)dst, bits, \
+                       words, params, s_in, s_out, nseg)
+    if (f16) {
+        if (xedge) { if (force) LBM_L(_Float16, 1, true); else LBM_L(_Float16, 1, false); }
+        else { if (force) LBM_L(_Float16, 0, true); else LBM_L(_Float16, 0, false); }
+    } else {
+        if (xedge) { if (force) LBM_L(float, 1, true); else LBM_L(float, 1, false); }
+        else { if (force) LBM_L(float, 0, true); else LBM_L(float, 0, false); }
+    }
+#undef LBM_L
+    return hipGetLastError();
 }
 
 hipError_t launch_build_solid_bits(const uint8_t* mat, int nx, int ny, int periodic_x, uint32_t words,

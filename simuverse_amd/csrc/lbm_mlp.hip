@@ -1,0 +1,588 @@
+// lbm_mlp.hip -- the learned collision correction's network: forward,
+// backward and Adam kernels (binary32), the sample store, and its C ABI.
+//
+// No reference counterpart: jinleili/simuverse has no learned collision.
+// The network is 9 -> 64 -> 64 -> 9 with GELU (erf form) after both hidden
+// layers.  Parameters are one flat vector of 5385 floats:
+//     W1[64][9] b1[64] W2[64][64] b2[64] W3[9][64] b3[9]   (row = output)
+// The loss of a batch of B samples is L = 1/(9B) sum (y - t)^2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lbm.h"
+#include "lbm_internal.hpp"
+
+using namespace lbm;
+
+int lbm_fail(int code, const char* fmt, ...);
+
+#define MLP_TRY(expr)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess) return lbm_fail(LBM_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace lbm {
+
+// SplitMix64 finaliser: sample indices of a minibatch.
+__device__ inline uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// ---------------------------------------------------------------------------
+// Forward pass (binary32), one sample per thread, weights in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ params, const float* __restrict__ X,
+                                                     float* __restrict__ Y, int n)
+{
+    __shared__ float P[kMlpParams];
+    for (int k = threadIdx.x; k < kMlpParams; k += 256) P[k] = params[k];
+    __syncthreads();
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    float x[kMlpIn], y[kMlpOut];
+#pragma unroll
+    for (int i = 0; i < kMlpIn; ++i) x[i] = X[(size_t)s * kMlpIn + i];
+    mlp_forward_one(P, x, y);
+#pragma unroll
+    for (int m = 0; m < kMlpOut; ++m) Y[(size_t)s * kMlpOut + m] = y[m];
+}
+
+// ---------------------------------------------------------------------------
+// Loss gradient: a block of 256 threads takes 64 samples; thread (s, q)
+// = (tid % 64, tid / 64) computes hidden units 16q..16q+15 of sample s.
+// Activations are staged in LDS (rows padded to 65 floats), then the block
+// forms its weight-gradient partial as sample-sums over LDS.
+// ---------------------------------------------------------------------------
+constexpr int kTS = 64;       // samples per block
+constexpr int kPadH = kMlpH + 1;
+
+__global__ __launch_bounds__(256) void k_mlp_grad_partial(const float* __restrict__ params,
+                                                          const float* __restrict__ X, const float* __restrict__ T,
+                                                          int n, int batch, uint64_t seed, int use_index,
+                                                          float* __restrict__ partial, double* __restrict__ loss_part)
+{
+    __shared__ float P[kMlpParams];
+    __shared__ float xs[kTS][kMlpIn], dy[kTS][kMlpOut];
+    __shared__ float A1[kTS][kPadH], H1[kTS][kPadH], A2[kTS][kPadH], H2[kTS][kPadH], D1[kTS][kPadH],
+        D2[kTS][kPadH];
+    __shared__ double lsum[kTS];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < kMlpParams; k += 256) P[k] = params[k];
+    const int s = tid & (kTS - 1), q = tid >> 6;
+    const int g = blockIdx.x * kTS + s;  // position in the batch
+    const bool live = g < batch;
+    // sample index: the batch position itself, or a hashed draw from n
+    const int idx = !live ? 0 : (use_index ? (int)(mix64(seed + 0x9E3779B97F4A7C15ull * (uint64_t)(g + 1)) % (uint64_t)n) : g);
+    if (q == 0) {
+#pragma unroll
+        for (int i = 0; i < kMlpIn; ++i) xs[s][i] = live ? X[(size_t)idx * kMlpIn + i] : 0.0f;
+    }
+    __syncthreads();
+    // layer 1
+    for (int jj = 0; jj < 16; ++jj) {
+        const int j = 16 * q + jj;
+        float a = P[kOffB1 + j];
+#pragma unroll
+        for (int i = 0; i < kMlpIn; ++i) a += P[kOffW1 + j * kMlpIn + i] * xs[s][i];
+        A1[s][j] = a;
+        H1[s][j] = gelu(a);
+    }
+    __syncthreads();
+    // layer 2
+    for (int jj = 0; jj < 16; ++jj) {
+        const int j = 16 * q + jj;
+        float a = P[kOffB2 + j];
+        for (int k = 0; k < kMlpH; ++k) a += P[kOffW2 + j * kMlpH + k] * H1[s][k];
+        A2[s][j] = a;
+        H2[s][j] = gelu(a);
+    }
+    __syncthreads();
+    // output, loss and dL/dy
+    if (q == 0) {
+        double l = 0.0;
+        const float inv = 2.0f / (9.0f * (float)batch);
+#pragma unroll
+        for (int m = 0; m < kMlpOut; ++m) {
+            float y = P[kOffB3 + m];
+            for (int k = 0; k < kMlpH; ++k) y += P[kOffW3 + m * kMlpH + k] * H2[s][k];
+            const float r = live ? y - T[(size_t)idx * kMlpOut + m] : 0.0f;
+            l += (double)r * (double)r;
+            dy[s][m] = r * inv;
+        }
+        lsum[s] = l;
+    }
+    __syncthreads();
+    for (int jj = 0; jj < 16; ++jj) {
+        const int j = 16 * q + jj;
+        float d = 0.0f;
+#pragma unroll
+        for (int m = 0; m < kMlpOut; ++m) d += P[kOffW3 + m * kMlpH + j] * dy[s][m];
+        D2[s][j] = d * gelu_grad(A2[s][j]);
+    }
+    __syncthreads();
+    for (int jj = 0; jj < 16; ++jj) {
+        const int j = 16 * q + jj;
+        float d = 0.0f;
+        for (int k = 0; k < kMlpH; ++k) d += P[kOffW2 + k * kMlpH + j] * D2[s][k];
+        D1[s][j] = d * gelu_grad(A1[s][j]);
+    }
+    __syncthreads();
+    // weight-gradient partials of this block: sums over its samples
+    float* out = partial + (size_t)blockIdx.x * kMlpParams;
+    for (int p = tid; p < kMlpParams; p += 256) {
+        float acc = 0.0f;
+        if (p < kOffB1) {
+            const int j = p / kMlpIn, i = p % kMlpIn;
+            for (int t = 0; t < kTS; ++t) acc += D1[t][j] * xs[t][i];
+        } else if (p < kOffW2) {
+            const int j = p - kOffB1;
+            for (int t = 0; t < kTS; ++t) acc += D1[t][j];
+        } else if (p < kOffB2) {
+            const int j = (p - kOffW2) / kMlpH, k = (p - kOffW2) % kMlpH;
+            for (int t = 0; t < kTS; ++t) acc += D2[t][j] * H1[t][k];
+        } else if (p < kOffW3) {
+            const int j = p - kOffB2;
+            for (int t = 0; t < kTS; ++t) acc += D2[t][j];
+        } else if (p < kOffB3) {
+            const int m = (p - kOffW3) / kMlpH, k = (p - kOffW3) % kMlpH;
+            for (int t = 0; t < kTS; ++t) acc += dy[t][m] * H2[t][k];
+        } else {
+            const int m = p - kOffB3;
+            for (int t = 0; t < kTS; ++t) acc += dy[t][m];
+        }
+        out[p] = acc;
+    }
+    if (tid == 0) {
+        double l = 0.0;
+        for (int t = 0; t < kTS; ++t) l += lsum[t];
+        loss_part[blockIdx.x] = l;
+    }
+}
+
+// grad[p] = sum over blocks of partial[b][p] in block order.
+__global__ __launch_bounds__(256) void k_mlp_grad_reduce(const float* __restrict__ partial, int nblocks,
+                                                         float* __restrict__ grad, const double* __restrict__ loss_part,
+                                                         double* __restrict__ loss, int batch)
+{
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p < kMlpParams) {
+        float acc = 0.0f;
+        for (int b = 0; b < nblocks; ++b) acc += partial[(size_t)b * kMlpParams + p];
+        grad[p] = acc;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double l = 0.0;
+        for (int b = 0; b < nblocks; ++b) l += loss_part[b];
+        *loss = l / (9.0 * (double)batch);
+    }
+}
+
+// Adam (Kingma & Ba 2015, Algorithm 1) on the binary32 master parameters.
+__global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g,
+                                                  float* __restrict__ m, float* __restrict__ v, float lr, float b1,
+                                                  float b2, float eps, float c1, float c2)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= kMlpParams) return;
+    const float gk = g[k];
+    const float mk = b1 * m[k] + (1.0f - b1) * gk;
+    const float vk = b2 * v[k] + (1.0f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// Samples from a pair of coarse pre-collision states (state_t, state_t1,
+// dense host layout f[i*n + c], binary32 f):
+//   features  x_i = (f_i - feq_i) / (w_i s_in)          at cell c, time t
+//   target    t_i = P(f_i(c + e_i, t+1) - BGK_i(f(c, t))) / (w_i s_out)
+// where P removes the mass and momentum of a correction.  Cells that are
+// solid or have a solid neighbour are skipped; y and x are periodic.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mlp_transition(const float* __restrict__ s0, const float* __restrict__ s1,
+                                                        const uint8_t* __restrict__ solid, int nx, int ny, float omega,
+                                                        float gx, float gy, float s_in, float s_out, uint32_t stride,
+                                                        uint32_t offset, int max_samples, int* __restrict__ count,
+                                                        float* __restrict__ X, float* __restrict__ T)
+{
+    const uint32_t n = (uint32_t)nx * (uint32_t)ny;
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    const uint64_t c64 = (uint64_t)k * stride + offset;
+    if (c64 >= n) return;
+    const uint32_t c = (uint32_t)c64;
+    const int y = (int)(c / (uint32_t)nx), x = (int)(c % (uint32_t)nx);
+    for (int dy2 = -1; dy2 <= 1; ++dy2)
+        for (int dx2 = -1; dx2 <= 1; ++dx2) {
+            const int xx = (x + dx2 + nx) % nx, yy = (y + dy2 + ny) % ny;
+            if (solid[(uint32_t)yy * nx + xx]) return;
+        }
+    float f[9], g[9];
+    float rho = 0.0f, jx = 0.0f, jy = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        f[i] = s0[(size_t)i * n + c];
+        rho += f[i];
+        jx += (float)ex_of(i) * f[i];
+        jy += (float)ey_of(i) * f[i];
+    }
+    const float ux = (jx + 0.5f * gx) / rho, uy = (jy + 0.5f * gy) / rho;
+    const float usq = 1.5f * (ux * ux + uy * uy);
+    const float fcoef = 1.0f - 0.5f * omega;
+    float xin[9], d[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float exf = (float)ex_of(i), eyf = (float)ey_of(i);
+        const float eu = exf * ux + eyf * uy;
+        const float feq = w_phys(i) * rho * (1.0f + 3.0f * eu + 4.5f * eu * eu - usq);
+        xin[i] = (f[i] - feq) / (w_phys(i) * s_in);
+        const float fx = 3.0f * (exf - ux) + 9.0f * eu * exf, fy = 3.0f * (eyf - uy) + 9.0f * eu * eyf;
+        const float bgk = f[i] - omega * (f[i] - feq) + fcoef * w_phys(i) * (fx * gx + fy * gy);
+        const int tx = (x + ex_of(i) + nx) % nx, ty = (y + ey_of(i) + ny) % ny;
+        d[i] = s1[(size_t)i * n + (uint32_t)ty * nx + tx] - bgk;
+    }
+    float a = 0.0f, bx = 0.0f, by = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        a += d[i];
+        bx += (float)ex_of(i) * d[i];
+        by += (float)ey_of(i) * d[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        g[i] = d[i] - w_phys(i) * (a + 3.0f * ((float)ex_of(i) * bx + (float)ey_of(i) * by));
+        g[i] /= (w_phys(i) * s_out);
+    }
+    const int slot = atomicAdd(count, 1);
+    if (slot >= max_samples) return;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        X[(size_t)slot * 9 + i] = xin[i];
+        T[(size_t)slot * 9 + i] = g[i];
+    }
+}
+
+}  // namespace lbm
+
+struct lbm_mlp {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    float* params = nullptr;  // binary32 master copy
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    float* grad = nullptr;
+    float* partial = nullptr;  // per-block gradient partials
+    double* loss_part = nullptr;
+    double* loss = nullptr;
+    int partial_blocks = 0;
+    float* X = nullptr;  // samples
+    float* T = nullptr;
+    int* count = nullptr;
+    int capacity = 0;
+    int samples = 0;
+    int adam_step = 0;
+    float s_in = 1.0f, s_out = 1.0f;
+};
+
+namespace {
+
+int mlp_check(lbm_mlp* m)
+{
+    if (!m) return lbm_fail(LBM_ERR_INVALID, "null network");
+    MLP_TRY(hipSetDevice(m->dev));
+    return LBM_OK;
+}
+
+void mlp_free(lbm_mlp* m)
+{
+    for (void* p : {(void*)m->params, (void*)m->adam_m, (void*)m->adam_v, (void*)m->grad, (void*)m->partial,
+                    (void*)m->loss_part, (void*)m->loss, (void*)m->X, (void*)m->T, (void*)m->count})
+        if (p) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+// Gradient of the loss over `batch` samples (the first `batch` stored
+// samples, or hashed draws when use_index) into m->grad; loss into m->loss.
+int mlp_grad(lbm_mlp* m, const float* X, const float* T, int n, int batch, uint64_t seed, int use_index)
+{
+    const int nb = (batch + kTS - 1) / kTS;
+    if (nb > m->partial_blocks) {
+        if (m->partial) (void)hipFree(m->partial);
+        if (m->loss_part) (void)hipFree(m->loss_part);
+        m->partial = nullptr;
+        m->loss_part = nullptr;
+        MLP_TRY(hipMalloc(&m->partial, (size_t)nb * kMlpParams * sizeof(float)));
+        MLP_TRY(hipMalloc(&m->loss_part, (size_t)nb * sizeof(double)));
+        m->partial_blocks = nb;
+    }
+    hipLaunchKernelGGL(k_mlp_grad_partial, dim3(nb), dim3(256), 0, m->stream, m->params, X, T, n, batch, seed,
+                       use_index, m->partial, m->loss_part);
+    MLP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_mlp_grad_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, m->stream, m->partial, nb,
+                       m->grad, m->loss_part, m->loss, batch);
+    MLP_TRY(hipGetLastError());
+    return LBM_OK;
+}
+
+}  // namespace
+
+// Device pointer to the binary32 parameters and the two scales of a
+// network (used by the learned collision step).
+const float* lbm_mlp_device_params(const lbm_mlp* m) { return m->params; }
+void lbm_mlp_scales(const lbm_mlp* m, float* s_in, float* s_out)
+{
+    *s_in = m->s_in;
+    *s_out = m->s_out;
+}
+
+extern "C" {
+
+int lbm_mlp_param_count(void) { return kMlpParams; }
+
+int lbm_mlp_create(uint64_t seed, int capacity, lbm_mlp** out)
+{
+    if (!out) return lbm_fail(LBM_ERR_INVALID, "null out");
+    *out = nullptr;
+    if (capacity < 0) return lbm_fail(LBM_ERR_INVALID, "capacity must be >= 0");
+    lbm_mlp* m = new lbm_mlp();
+    hipError_t e = hipGetDevice(&m->dev);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    const size_t pb = kMlpParams * sizeof(float);
+    if (e == hipSuccess) e = hipMalloc(&m->params, pb);
+    if (e == hipSuccess) e = hipMalloc(&m->adam_m, pb);
+    if (e == hipSuccess) e = hipMalloc(&m->adam_v, pb);
+    if (e == hipSuccess) e = hipMalloc(&m->grad, pb);
+    if (e == hipSuccess) e = hipMalloc(&m->loss, sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&m->count, sizeof(int));
+    if (e == hipSuccess && capacity > 0) e = hipMalloc(&m->X, (size_t)capacity * kMlpIn * sizeof(float));
+    if (e == hipSuccess && capacity > 0) e = hipMalloc(&m->T, (size_t)capacity * kMlpOut * sizeof(float));
+    if (e != hipSuccess) {
+        mlp_free(m);
+        return lbm_fail(LBM_ERR_OOM, "network allocation: %s", hipGetErrorString(e));
+    }
+    m->capacity = capacity;
+    // Glorot-uniform weights from SplitMix64(seed), zero biases.
+    std::vector<float> p(kMlpParams, 0.0f);
+    uint64_t st = seed;
+    auto uni = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto fill = [&](int off, int fan_out, int fan_in) {
+        const double a = std::sqrt(6.0 / (fan_in + fan_out));
+        for (int k = 0; k < fan_out * fan_in; ++k) p[off + k] = (float)((2.0 * uni() - 1.0) * a);
+    };
+    fill(kOffW1, kMlpH, kMlpIn);
+    fill(kOffW2, kMlpH, kMlpH);
+    fill(kOffW3, kMlpOut, kMlpH);
+    e = hipMemcpy(m->params, p.data(), pb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(m->adam_m, 0, pb);
+    if (e == hipSuccess) e = hipMemset(m->adam_v, 0, pb);
+    if (e != hipSuccess) {
+        mlp_free(m);
+        return lbm_fail(LBM_ERR_HIP, "network init: %s", hipGetErrorString(e));
+    }
+    *out = m;
+    return LBM_OK;
+}
+
+int lbm_mlp_destroy(lbm_mlp* m)
+{
+    if (!m) return LBM_OK;
+    (void)hipSetDevice(m->dev);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    mlp_free(m);
+    return LBM_OK;
+}
+
+int lbm_mlp_set_params(lbm_mlp* m, const float* params)
+{
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (!params) return lbm_fail(LBM_ERR_INVALID, "null params");
+    MLP_TRY(hipMemcpyAsync(m->params, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    MLP_TRY(hipMemsetAsync(m->adam_m, 0, kMlpParams * sizeof(float), m->stream));
+    MLP_TRY(hipMemsetAsync(m->adam_v, 0, kMlpParams * sizeof(float), m->stream));
+    MLP_TRY(hipStreamSynchronize(m->stream));
+    m->adam_step = 0;
+    return LBM_OK;
+}
+
+int lbm_mlp_get_params(lbm_mlp* m, float* params)
+{
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (!params) return lbm_fail(LBM_ERR_INVALID, "null params");
+    MLP_TRY(hipMemcpyAsync(params, m->params, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    MLP_TRY(hipStreamSynchronize(m->stream));
+    return LBM_OK;
+}
+
+int lbm_mlp_set_scales(lbm_mlp* m, float in_scale, float out_scale)
+{
+    if (!m) return lbm_fail(LBM_ERR_INVALID, "null network");
+    if (!(in_scale > 0.0f) || !(out_scale > 0.0f) || !std::isfinite(in_scale) || !std::isfinite(out_scale))
+        return lbm_fail(LBM_ERR_INVALID, "scales must be finite and > 0");
+    m->s_in = in_scale;
+    m->s_out = out_scale;
+    return LBM_OK;
+}
+
+int lbm_mlp_get_scales(lbm_mlp* m, float* in_scale, float* out_scale)
+{
+    if (!m || !in_scale || !out_scale) return lbm_fail(LBM_ERR_INVALID, "null argument");
+    *in_scale = m->s_in;
+    *out_scale = m->s_out;
+    return LBM_OK;
+}
+
+int lbm_mlp_forward(lbm_mlp* m, const float* x, int n, float* y)
+{
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (!x || !y || n < 0) return lbm_fail(LBM_ERR_INVALID, "bad arguments");
+    if (n == 0) return LBM_OK;
+    float *dx = nullptr, *dy = nullptr;
+    MLP_TRY(hipMalloc(&dx, (size_t)n * kMlpIn * sizeof(float)));
+    hipError_t e = hipMalloc(&dy, (size_t)n * kMlpOut * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpyAsync(dx, x, (size_t)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_mlp_forward, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->params, dx, dy, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(y, dy, (size_t)n * kMlpOut * sizeof(float), hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    if (e != hipSuccess) return lbm_fail(LBM_ERR_HIP, "network forward: %s", hipGetErrorString(e));
+    return LBM_OK;
+}
+
+int lbm_mlp_loss_grad(lbm_mlp* m, const float* x, const float* t, int n, float* grad, double* loss)
+{
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (!x || !t || n < 1) return lbm_fail(LBM_ERR_INVALID, "bad arguments");
+    float *dx = nullptr, *dt = nullptr;
+    MLP_TRY(hipMalloc(&dx, (size_t)n * kMlpIn * sizeof(float)));
+    hipError_t e = hipMalloc(&dt, (size_t)n * kMlpOut * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpyAsync(dx, x, (size_t)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dt, t, (size_t)n * kMlpOut * sizeof(float), hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) {
+        rc = mlp_grad(m, dx, dt, n, n, 0, 0);
+        if (rc != LBM_OK) e = hipErrorUnknown;
+    }
+    if (e == hipSuccess && grad)
+        e = hipMemcpyAsync(grad, m->grad, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess && loss) e = hipMemcpyAsync(loss, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(dx);
+    if (dt) (void)hipFree(dt);
+    if (rc != LBM_OK) return rc;
+    if (e != hipSuccess) return lbm_fail(LBM_ERR_HIP, "network gradient: %s", hipGetErrorString(e));
+    return LBM_OK;
+}
+
+int lbm_mlp_clear_samples(lbm_mlp* m)
+{
+    if (!m) return lbm_fail(LBM_ERR_INVALID, "null network");
+    m->samples = 0;
+    return LBM_OK;
+}
+
+int lbm_mlp_sample_count(lbm_mlp* m, int* n)
+{
+    if (!m || !n) return lbm_fail(LBM_ERR_INVALID, "null argument");
+    *n = m->samples;
+    return LBM_OK;
+}
+
+int lbm_mlp_add_samples(lbm_mlp* m, const float* x, const float* t, int n)
+{
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (!x || !t || n < 0 || m->samples + n > m->capacity)
+        return lbm_fail(LBM_ERR_INVALID, "samples exceed the capacity of %d", m->capacity);
+    MLP_TRY(hipMemcpyAsync(m->X + (size_t)m->samples * kMlpIn, x, (size_t)n * kMlpIn * sizeof(float),
+                           hipMemcpyHostToDevice, m->stream));
+    MLP_TRY(hipMemcpyAsync(m->T + (size_t)m->samples * kMlpOut, t, (size_t)n * kMlpOut * sizeof(float),
+                           hipMemcpyHostToDevice, m->stream));
+    MLP_TRY(hipStreamSynchronize(m->stream));
+    m->samples += n;
+    return LBM_OK;
+}
+
+int lbm_mlp_add_transition(lbm_mlp* m, const float* state_t, const float* state_t1, const uint8_t* solid, int nx,
+                           int ny, float tau, float force_x, float force_y, int stride, int offset, int* added)
+{
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (!state_t || !state_t1 || !solid || nx < 3 || ny < 3 || stride < 1 || offset < 0 || !(tau > 0.5f))
+        return lbm_fail(LBM_ERR_INVALID, "bad arguments");
+    const size_t n = (size_t)nx * (size_t)ny;
+    float *s0 = nullptr, *s1 = nullptr;
+    uint8_t* sd = nullptr;
+    MLP_TRY(hipMalloc(&s0, 9 * n * sizeof(float)));
+    hipError_t e = hipMalloc(&s1, 9 * n * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&sd, n);
+    if (e == hipSuccess) e = hipMemcpyAsync(s0, state_t, 9 * n * sizeof(float), hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s1, state_t1, 9 * n * sizeof(float), hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sd, solid, n, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(m->count, &m->samples, sizeof(int), hipMemcpyHostToDevice, m->stream);
+    const size_t cand = (n - (size_t)std::min<int>(offset, (int)n) + stride - 1) / stride;
+    if (e == hipSuccess && cand > 0) {
+        hipLaunchKernelGGL(k_mlp_transition, dim3((unsigned)((cand + 255) / 256)), dim3(256), 0, m->stream, s0, s1, sd,
+                           nx, ny, 1.0f / tau, force_x, force_y, m->s_in, m->s_out, (uint32_t)stride, (uint32_t)offset,
+                           m->capacity, m->count, m->X, m->T);
+        e = hipGetLastError();
+    }
+    int total = m->samples;
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, m->count, sizeof(int), hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(s0);
+    if (s1) (void)hipFree(s1);
+    if (sd) (void)hipFree(sd);
+    if (e != hipSuccess) return lbm_fail(LBM_ERR_HIP, "transition samples: %s", hipGetErrorString(e));
+    total = std::min(total, m->capacity);
+    if (added) *added = total - m->samples;
+    m->samples = total;
+    return LBM_OK;
+}
+
+int lbm_mlp_train(lbm_mlp* m, int steps, int batch, float lr, uint64_t seed, double* last_loss)
+{
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (steps < 0 || batch < 1 || !(lr > 0.0f)) return lbm_fail(LBM_ERR_INVALID, "bad arguments");
+    if (m->samples < 1) return lbm_fail(LBM_ERR_INVALID, "no samples to train on");
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int s = 0; s < steps; ++s) {
+        rc = mlp_grad(m, m->X, m->T, m->samples, batch, seed + 0xD1B54A32D192ED03ull * (uint64_t)(m->adam_step + 1), 1);
+        if (rc != LBM_OK) return rc;
+        ++m->adam_step;
+        const float c1 = 1.0f - std::pow(b1, (float)m->adam_step);
+        const float c2 = 1.0f - std::pow(b2, (float)m->adam_step);
+        hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, m->stream, m->params, m->grad,
+                           m->adam_m, m->adam_v, lr, b1, b2, eps, c1, c2);
+        MLP_TRY(hipGetLastError());
+    }
+    if (last_loss) MLP_TRY(hipMemcpyAsync(last_loss, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    MLP_TRY(hipStreamSynchronize(m->stream));
+    return LBM_OK;
+}
+
+}  // extern "C"

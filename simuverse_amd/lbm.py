@@ -108,6 +108,25 @@ _SIGNATURES = [
     ("lbm_batch_get_variant", C.c_int, [_P, C.POINTER(C.c_int)]),
     ("lbm_batch_set_graph_chunk", C.c_int, [_P, C.c_int]),
     ("lbm_batch_autotune", C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    ("lbm_mlp_param_count", C.c_int, []),
+    ("lbm_mlp_create", C.c_int, [C.c_uint64, C.c_int, C.POINTER(_P)]),
+    ("lbm_mlp_destroy", C.c_int, [_P]),
+    ("lbm_mlp_set_params", C.c_int, [_P, _P]),
+    ("lbm_mlp_get_params", C.c_int, [_P, _P]),
+    ("lbm_mlp_set_scales", C.c_int, [_P, C.c_float, C.c_float]),
+    ("lbm_mlp_get_scales", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("lbm_mlp_forward", C.c_int, [_P, _P, C.c_int, _P]),
+    ("lbm_mlp_loss_grad", C.c_int, [_P, _P, _P, C.c_int, _P, C.POINTER(C.c_double)]),
+    ("lbm_mlp_clear_samples", C.c_int, [_P]),
+    ("lbm_mlp_sample_count", C.c_int, [_P, C.POINTER(C.c_int)]),
+    ("lbm_mlp_add_samples", C.c_int, [_P, _P, _P, C.c_int]),
+    ("lbm_mlp_add_transition", C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                         C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    ("lbm_mlp_train", C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_uint64, C.POINTER(C.c_double)]),
+    ("lbm_set_learned_collision", C.c_int, [_P, _P]),
+    ("lbm_health", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_int)]),
+    ("lbm_run_guarded", C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_int)]),
+    ("lbm_restrict4", C.c_int, [_P, C.c_float, C.c_float, C.c_float, _P, _P, _P, _P]),
 ]
 
 
@@ -306,6 +325,31 @@ class Lattice:
     def set_graph_chunk(self, chunk: int) -> None:
         _check(self._lib.lbm_set_graph_chunk(self._h, chunk))
 
+    # -- learned collision ------------------------------------------------
+    def set_learned_collision(self, net: "Network | None") -> None:
+        _check(self._lib.lbm_set_learned_collision(self._h, net.handle if net is not None else None))
+        self._net = net  # keep the network alive while attached
+
+    def health(self) -> tuple[float, bool]:
+        u, f = C.c_float(), C.c_int()
+        _check(self._lib.lbm_health(self._h, C.byref(u), C.byref(f)))
+        return u.value, bool(f.value)
+
+    def run_guarded(self, nsteps: int, chunk: int = 100, umax: float = 0.3) -> int:
+        falls = C.c_int()
+        _check(self._lib.lbm_run_guarded(self._h, nsteps, chunk, umax, C.byref(falls)))
+        return falls.value
+
+    def restrict4(self, tau_coarse: float, force_coarse: tuple[float, float] = (0.0, 0.0)):
+        """(coarse pre-collision f (9, ny/4, nx/4), coarse solid, ux, uy)."""
+        cy, cx = self.ny // 4, self.nx // 4
+        f = np.empty((9, cy, cx), np.float32)
+        s = np.empty((cy, cx), np.uint8)
+        ux, uy = np.empty((cy, cx), np.float32), np.empty((cy, cx), np.float32)
+        _check(self._lib.lbm_restrict4(self._h, tau_coarse, force_coarse[0], force_coarse[1], _ptr(f), _ptr(s),
+                                       _ptr(ux), _ptr(uy)))
+        return f, s.astype(bool), ux, uy
+
     # -- outputs ----------------------------------------------------------
     def read_macro(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         rho, ux, uy = (np.empty((self.ny, self.nx), np.float32) for _ in range(3))
@@ -414,6 +458,93 @@ class Batch:
         v, ms = C.c_int(), C.c_double()
         _check(self._lib.lbm_batch_autotune(self._h, steps, C.byref(v), C.byref(ms)))
         return v.value, ms.value
+
+
+class Network:
+    """The learned collision correction's network (9 -> 64 -> 64 -> 9)."""
+
+    def __init__(self, seed: int = 1, capacity: int = 0):
+        self._lib = load_library()
+        h = C.c_void_p()
+        _check(self._lib.lbm_mlp_create(seed, capacity, C.byref(h)))
+        self._h = h
+        self.nparams = self._lib.lbm_mlp_param_count()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _check(self._lib.lbm_mlp_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.nparams, np.float32)
+        _check(self._lib.lbm_mlp_get_params(self._h, _ptr(p)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        _check(self._lib.lbm_mlp_set_params(self._h, _ptr(p)))
+
+    def set_scales(self, s_in: float, s_out: float) -> None:
+        _check(self._lib.lbm_mlp_set_scales(self._h, s_in, s_out))
+
+    def scales(self) -> tuple[float, float]:
+        a, b = C.c_float(), C.c_float()
+        _check(self._lib.lbm_mlp_get_scales(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 9)
+        y = np.empty_like(x)
+        _check(self._lib.lbm_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(y)))
+        return y
+
+    def loss_grad(self, x: np.ndarray, t: np.ndarray) -> tuple[float, np.ndarray]:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 9)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 9)
+        g = np.empty(self.nparams, np.float32)
+        loss = C.c_double()
+        _check(self._lib.lbm_mlp_loss_grad(self._h, _ptr(x), _ptr(t), x.shape[0], _ptr(g), C.byref(loss)))
+        return loss.value, g
+
+    def clear_samples(self) -> None:
+        _check(self._lib.lbm_mlp_clear_samples(self._h))
+
+    def sample_count(self) -> int:
+        n = C.c_int()
+        _check(self._lib.lbm_mlp_sample_count(self._h, C.byref(n)))
+        return n.value
+
+    def add_samples(self, x: np.ndarray, t: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 9)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 9)
+        _check(self._lib.lbm_mlp_add_samples(self._h, _ptr(x), _ptr(t), x.shape[0]))
+
+    def add_transition(self, s0: np.ndarray, s1: np.ndarray, solid: np.ndarray, tau: float,
+                       force: tuple[float, float] = (0.0, 0.0), stride: int = 1, offset: int = 0) -> int:
+        ny, nx = solid.shape
+        s0 = np.ascontiguousarray(s0, dtype=np.float32).reshape(9, ny, nx)
+        s1 = np.ascontiguousarray(s1, dtype=np.float32).reshape(9, ny, nx)
+        sd = np.ascontiguousarray(solid, dtype=np.uint8)
+        added = C.c_int()
+        _check(self._lib.lbm_mlp_add_transition(self._h, _ptr(s0), _ptr(s1), _ptr(sd), nx, ny, tau, force[0],
+                                                 force[1], stride, offset, C.byref(added)))
+        return added.value
+
+    def train(self, steps: int, batch: int, lr: float, seed: int = 7) -> float:
+        loss = C.c_double()
+        _check(self._lib.lbm_mlp_train(self._h, steps, batch, lr, seed, C.byref(loss)))
+        return loss.value
 
 
 def batch_variant_count() -> int:

@@ -1,0 +1,223 @@
+"""Learned collision correction on the GPU, through the C ABI.
+
+* the network's binary32 forward and loss gradient against the binary64
+  oracle (oracle/mlp_oracle.c);
+* training (device forward/backward/Adam) reduces the loss;
+* the projected correction leaves mass and momentum exactly as plain BGK
+  does;
+* a network whose output blows up makes lbm_run_guarded fall back to plain
+  BGK, and the result equals the plain run bit for bit;
+* the 4x restriction keeps block mass and momentum;
+* on a coarse lattice the trained network brings the velocity closer to the
+  restricted fine solution than plain BGK does.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from simuverse_amd import lbm as L
+from simuverse_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gfx950():
+    assert L.device_count() >= 1, "no gfx950 device visible"
+
+
+def random_params(seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(5385) * scale).astype(np.float32)
+
+
+def test_forward_matches_fp64_oracle():
+    print("case: network forward vs fp64", flush=True)
+    net = L.Network(seed=3)
+    p = random_params(1)
+    net.set_params(p)
+    x = np.random.default_rng(2).standard_normal((1000, 9)).astype(np.float32)
+    y = net.forward(x)
+    ref = O.mlp_forward(p.astype(np.float64), x.astype(np.float64))
+    assert np.max(np.abs(y - ref)) < 2e-5 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_gradient_matches_fp64_oracle():
+    print("case: network gradient vs fp64", flush=True)
+    net = L.Network(seed=3)
+    p = random_params(4)
+    net.set_params(p)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((300, 9)).astype(np.float32)
+    t = rng.standard_normal((300, 9)).astype(np.float32)
+    loss, g = net.loss_grad(x, t)
+    lref, gref = O.mlp_loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    assert abs(loss - lref) < 1e-5 * lref
+    assert np.max(np.abs(g - gref)) < 1e-4 * np.max(np.abs(gref))
+    # every parameter block carries gradient
+    for a, b in ((0, 576), (576, 640), (640, 4736), (4736, 4800), (4800, 5376), (5376, 5385)):
+        assert np.linalg.norm(g[a:b] - gref[a:b]) <= 1e-4 * np.linalg.norm(gref[a:b]) + 1e-12
+
+
+def test_training_reduces_loss():
+    print("case: network training", flush=True)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((20000, 9)).astype(np.float32)
+    A = rng.standard_normal((9, 9)) * 0.4
+    t = np.tanh(x @ A).astype(np.float32)
+    net = L.Network(seed=11, capacity=20000)
+    net.add_samples(x, t)
+    l0, _ = net.loss_grad(x[:4096], t[:4096])
+    net.train(steps=400, batch=1024, lr=3e-3, seed=1)
+    l1, _ = net.loss_grad(x[:4096], t[:4096])
+    print(f"loss {l0:.4e} -> {l1:.4e}", flush=True)
+    assert l1 < 0.1 * l0
+
+
+def test_correction_conserves_mass_and_momentum():
+    print("case: learned correction conserves", flush=True)
+    nx, ny = 96, 64
+    net = L.Network(seed=5)
+    net.set_params(random_params(7, 0.2))
+    net.set_scales(1e-3, 1e-4)
+    f0 = W.rest_state(nx, ny, amplitude=1e-3, seed=3).astype(np.float32)
+    lat = L.Lattice(nx, ny, 0.6)
+    lat.write_populations(f0, 0)
+    lat.set_learned_collision(net)
+    r0 = lat.reduce()
+    lat.run(50)
+    r1 = lat.reduce()
+    assert abs(r1[0] - r0[0]) < 1e-6 * r0[0]
+    assert np.all(np.abs(r1[1:3] - r0[1:3]) < 1e-6)
+    # and the correction did change the state
+    plain = L.Lattice(nx, ny, 0.6)
+    plain.write_populations(f0, 0)
+    plain.run(50)
+    assert not np.array_equal(lat.read_populations(), plain.read_populations())
+
+
+def test_blow_up_falls_back_to_bgk():
+    print("case: guarded run fallback", flush=True)
+    nx, ny = 64, 48
+    solid = W.cylinder_channel_solid(nx, ny, cx=20, cy=24, r=5)
+    f0 = W.rest_state(nx, ny, solid, amplitude=1e-4).astype(np.float32)
+    net = L.Network(seed=8)
+    net.set_params(random_params(9, 0.3) * 1e3)  # a network that explodes
+    net.set_scales(1e-3, 1.0)
+    lat = L.Lattice(nx, ny, 0.6, x_edge=L.X_INLET_OUTLET, inlet_ux=0.05)
+    lat.write_materials(W.materials_from_solid(solid))
+    lat.write_populations(f0, 0)
+    lat.set_learned_collision(net)
+    falls = lat.run_guarded(200, chunk=50, umax=0.3)
+    assert falls == 4
+    plain = L.Lattice(nx, ny, 0.6, x_edge=L.X_INLET_OUTLET, inlet_ux=0.05)
+    plain.write_materials(W.materials_from_solid(solid))
+    plain.write_populations(f0, 0)
+    plain.run(200)
+    fl = ~solid
+    assert np.array_equal(lat.read_populations()[:, fl], plain.read_populations()[:, fl])
+    u, finite = lat.health()
+    assert finite and u < 0.3
+    # a healthy network never falls back
+    ok = L.Network(seed=8)
+    ok.set_params(np.zeros(5385, np.float32))
+    lat2 = L.Lattice(nx, ny, 0.6, x_edge=L.X_INLET_OUTLET, inlet_ux=0.05)
+    lat2.write_materials(W.materials_from_solid(solid))
+    lat2.write_populations(f0, 0)
+    lat2.set_learned_collision(ok)
+    assert lat2.run_guarded(200, chunk=50, umax=0.3) == 0
+
+
+def test_restriction_keeps_block_mass_and_momentum():
+    print("case: restriction", flush=True)
+    nx, ny = 128, 96
+    scene = W.periodic_disks(nx, ny, 0.05, 6, 10, seed=2)
+    g = (2e-5, 0.0)
+    lat = L.Lattice(nx, ny, 0.52, force=g)
+    lat.write_materials(W.materials_from_solid(scene.solid))
+    lat.run(300)
+    rho, ux, uy = lat.macro()
+    fc, sc, uxc, uyc = lat.restrict4(0.505, (4 * g[0], 0.0))
+    fl = ~scene.solid
+    blk = lambda a: a.reshape(ny // 4, 4, nx // 4, 4).sum(axis=(1, 3))
+    nfl = blk(fl.astype(float))
+    cf = ~sc
+    assert np.array_equal(sc, nfl < 8)
+    m = blk(np.where(fl, rho, 0.0))
+    np.testing.assert_allclose(fc.sum(axis=0)[cf], (m / nfl)[cf], rtol=2e-6)
+    np.testing.assert_allclose(uxc[cf], (blk(np.where(fl, rho * ux, 0.0)) / m)[cf], rtol=1e-4, atol=1e-9)
+    jx = sum(W.EX[i] * fc[i] for i in range(9))
+    np.testing.assert_allclose((jx + 2 * g[0]) / fc.sum(axis=0), uxc, rtol=1e-4, atol=1e-8)
+
+
+def rms_scales(s0, s1, solid, tau, g):
+    """Root-mean-square of the network input (f - feq)/w and of the
+    residual target / w over the fluid cells of one transition: the
+    normalisations that make both of order one."""
+    w = W.W_PHYS[:, None, None]
+    ex, ey = W.EX[:, None, None], W.EY[:, None, None]
+    f = s0.astype(np.float64)
+    rho = f.sum(0)
+    ux = ((ex * f).sum(0) + 0.5 * g[0]) / rho
+    uy = ((ey * f).sum(0) + 0.5 * g[1]) / rho
+    eu = ex * ux + ey * uy
+    feq = w * rho * (1 + 3 * eu + 4.5 * eu**2 - 1.5 * (ux**2 + uy**2))
+    om = 1.0 / tau
+    F = (1 - 0.5 * om) * w * ((3 * (ex - ux) + 9 * eu * ex) * g[0] + (3 * (ey - uy) + 9 * eu * ey) * g[1])
+    bgk = f - om * (f - feq) + F
+    nxt = np.stack([np.roll(np.roll(s1[i], -W.EX[i], 1), -W.EY[i], 0) for i in range(9)])
+    fl = ~solid
+    return float(np.sqrt(np.mean(((f - feq) / w)[:, fl] ** 2))), float(np.sqrt(np.mean(((nxt - bgk) / w)[:, fl] ** 2)))
+
+
+def coarse_from_rest(n, tau, g, solid, steps, net=None):
+    lat = L.Lattice(n, n, tau, force=g)
+    lat.write_materials(W.materials_from_solid(solid))
+    if net is not None:
+        lat.set_learned_collision(net)
+    lat.run(steps)
+    _, ux, uy = lat.macro()
+    return ux, uy
+
+
+def test_learned_coarse_beats_plain_coarse():
+    """Fine 256^2 (tau 0.52, 2x2 cylinder array r = 24, body force) from rest,
+    restricted every 4 fine steps to 64^2; the network is trained on those
+    transitions; coarse runs at tau 0.505 from rest with and without it are
+    compared with the restricted fine velocity."""
+    print("case: learned coarse vs plain coarse", flush=True)
+    nf, nc = 256, 64
+    tau_f, tau_c = 0.52, 0.505
+    gf = (4e-6, 0.0)
+    gc = (4 * gf[0], 0.0)
+    y, x = np.mgrid[0:nf, 0:nf]
+    solid_f = np.zeros((nf, nf), bool)
+    for cx in (64, 192):
+        for cy in (64, 192):
+            solid_f |= (x - cx) ** 2 + (y - cy) ** 2 <= 24 * 24
+    fine = L.Lattice(nf, nf, tau_f, force=gf)
+    fine.write_materials(W.materials_from_solid(solid_f))
+    T = 300
+    states, solid_c = [], None
+    for t in range(T + 1):
+        fc, solid_c, uxc, uyc = fine.restrict4(tau_c, gc)
+        states.append((fc, uxc, uyc))
+        fine.run(4)
+    net = L.Network(seed=21, capacity=400000)
+    s_in, s_out = rms_scales(states[T // 2][0], states[T // 2 + 1][0], solid_c, tau_c, gc)
+    net.set_scales(s_in, s_out)
+    for t in range(T):
+        net.add_transition(states[t][0], states[t + 1][0], solid_c, tau_c, gc, stride=3, offset=t % 3)
+    print(f"samples {net.sample_count()}, scales {s_in:.3e} {s_out:.3e}", flush=True)
+    loss = net.train(steps=1500, batch=4096, lr=2e-3, seed=3)
+    print(f"final training loss {loss:.4e}", flush=True)
+    ux_p, uy_p = coarse_from_rest(nc, tau_c, gc, solid_c, T)
+    ux_l, uy_l = coarse_from_rest(nc, tau_c, gc, solid_c, T, net)
+    _, uxr, uyr = states[T]
+    fl = ~solid_c
+    ref = np.hypot(uxr, uyr)[fl]
+    e_plain = np.linalg.norm(np.hypot(ux_p - uxr, uy_p - uyr)[fl]) / np.linalg.norm(ref)
+    e_learn = np.linalg.norm(np.hypot(ux_l - uxr, uy_l - uyr)[fl]) / np.linalg.norm(ref)
+    print(f"velocity L2 error vs restricted fine: plain {e_plain:.4e}, learned {e_learn:.4e}", flush=True)
+    assert np.isfinite(e_learn)
+    assert e_learn < e_plain
