@@ -146,6 +146,166 @@ def run_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def run_learned(args):
+    """Learned collision on a 4x coarser lattice against the fine solution.
+
+    Fine: n x n binary32, tau_f = 0.52 (nu = 1/150), periodic 4x4 array of
+    cylinders of radius n/64, started from a uniform flow U = 0.05 with a Guo
+    body force g_f holding it (Re = U D / nu = 0.05 * n/32 * 150, 1920 at
+    n = 8192).  Coarse: (n/4)^2, tau_c = 0.505 (nu_c = nu_f / 4, the same
+    flow in acoustic scaling), force 4 g_f, majority-solid restriction.
+
+    1. fine run of N_f steps (timed); every `pair_every` fine steps the state
+       and the one 4 fine steps later are restricted and their transition
+       goes into the network's sample store (inputs: non-equilibrium part of
+       the coarse populations; target: the residual of one coarse BGK step);
+    2. Adam training on those samples (binary32, device);
+    3. coarse runs of N_f / 4 steps from the restricted fine initial state,
+       plain BGK and BGK + the network on the matrix cores (timed);
+    4. velocity relative L2 error of each coarse run against the restricted
+       fine velocity at N_f, and time-to-solution ratios.
+    """
+    from simuverse_amd import lbm as L
+    from simuverse_amd import workloads as W
+
+    n = args.n if args.n != 4096 else 8192
+    nc = n // 4
+    tau_f, tau_c, U = 0.52, 0.505, 0.05
+    gf = args.learned_force
+    gc = (4 * gf, 0.0)
+    Nf = args.learned_fine_steps
+    solid = W.cylinder_array_solid(n, 4, n // 64)
+    log(f"learned: fine {n}^2 tau {tau_f}, coarse {nc}^2 tau {tau_c}, {Nf} fine steps, "
+        f"solid {solid.mean():.4f}, Re {U * (n // 32) / ((tau_f - 0.5) / 3):.0f}")
+    fine = L.Lattice(n, n, tau_f, force=(gf, 0.0))
+    fine.write_materials(W.materials_from_solid(solid))
+    fine.write_populations(W.equilibrium_state(n, n, U, 0.0, solid).astype(np.float32), 0)
+    del solid
+    fine.autotune(steps=10)
+    # the timed fine solution: the whole N_f steps
+    fc0, solid_c, _, _ = fine.restrict4(tau_c, gc)
+    ms_fine = fine.run_timed(Nf)
+    _, _, uxr, uyr = fine.restrict4(tau_c, gc)
+    log(f"fine: {Nf} steps {ms_fine:.1f} ms ({n * n * Nf / ms_fine / 1e3:.0f} MLUPS)")
+
+    # training data from a second pass over the first half of the window
+    t0 = time.perf_counter()
+    net = L.Network(seed=7, capacity=args.learned_capacity)
+    fine2 = L.Lattice(n, n, tau_f, force=(gf, 0.0))
+    solid = W.cylinder_array_solid(n, 4, n // 64)
+    fine2.write_materials(W.materials_from_solid(solid))
+    fine2.write_populations(W.equilibrium_state(n, n, U, 0.0, solid).astype(np.float32), 0)
+    del solid
+    fine2.set_variant(fine.variant())
+    done, pairs, scaled = 0, 0, False
+    stride = max(1, (nc * nc * (Nf // 2 // args.pair_every)) // args.learned_capacity + 1)
+    for t in range(0, Nf // 2, args.pair_every):
+        fine2.run(t - done)
+        done = t
+        a, _, _, _ = fine2.restrict4(tau_c, gc)
+        fine2.run(4)
+        done += 4
+        b, _, _, _ = fine2.restrict4(tau_c, gc)
+        if not scaled:
+            s_in, s_out = rms_scales(a, b, solid_c, tau_c, gc)
+            net.set_scales(s_in, s_out)
+            scaled = True
+        net.add_transition(a, b, solid_c, tau_c, gc, stride=stride, offset=pairs % stride)
+        pairs += 1
+    fine2.close()
+    t_collect = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = net.train(steps=args.learned_train_steps, batch=min(args.learned_batch, net.sample_count()), lr=2e-3,
+                     seed=3)
+    t_train = time.perf_counter() - t0
+    log(f"samples {net.sample_count()} from {pairs} transitions ({t_collect:.1f} s), "
+        f"train {args.learned_train_steps} steps ({t_train:.1f} s), loss {loss:.3e}")
+
+    def coarse(learned: bool):
+        lat = L.Lattice(nc, nc, tau_c, force=gc)
+        lat.write_materials(W.materials_from_solid(solid_c))
+        lat.write_populations(fc0, 0)
+        if learned:
+            lat.set_learned_collision(net)
+        else:
+            lat.autotune(steps=10)
+            lat.write_populations(fc0, 0)
+        ms = lat.run_timed(Nf // 4)
+        _, ux, uy = lat.macro()
+        u, finite = lat.health()
+        fl = ~solid_c
+        ref = np.hypot(uxr, uyr)[fl]
+        err = float(np.linalg.norm(np.hypot(ux - uxr, uy - uyr)[fl]) / np.linalg.norm(ref)) if finite else float("inf")
+        lat.close()
+        return ms, err
+
+    ms_plain, e_plain = coarse(False)
+    ms_learn, e_learn = coarse(True)
+    log(f"coarse plain: {ms_plain:.1f} ms, vel L2 {e_plain:.4e}; learned: {ms_learn:.1f} ms, vel L2 {e_learn:.4e}")
+    K = Nf // 4
+    line = {
+        "metric": METRIC,
+        "value": round(nc * nc * K / ms_learn / 1e3, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": 0,
+        "ms_per_step": round(ms_learn / K, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 lattice, bf16 network on matrix cores (f32 accumulate)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"d2q9_learned_{n}_to_{nc}",
+            "fine": {"n": n, "tau": tau_f, "steps": Nf, "ms": round(ms_fine, 3)},
+            "coarse": {"n": nc, "tau": tau_c, "steps": K},
+            "force_fine": gf, "U0": U, "cylinders": "4x4, r = n/64",
+            "training": {"pairs": pairs, "samples": net.sample_count(), "steps": args.learned_train_steps,
+                         "batch": min(args.learned_batch, net.sample_count()), "final_loss": loss,
+                         "collect_s": round(t_collect, 2), "train_s": round(t_train, 2)},
+            "vel_l2_vs_restricted_fine": {"plain": e_plain, "learned": e_learn},
+            "coarse_ms": {"plain": round(ms_plain, 3), "learned": round(ms_learn, 3)},
+            "time_to_solution_speedup": {"learned_vs_fine": round(ms_fine / ms_learn, 2),
+                                         "learned_incl_training_vs_fine":
+                                             round(ms_fine / (ms_learn + 1e3 * (t_collect + t_train)), 3)},
+        },
+        "roofline": {"bound": "hbm", "achieved": round(72 * nc * nc * K / ms_learn / 1e6, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(72 * nc * nc * K / ms_learn / 1e6 / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def rms_scales(s0, s1, solid, tau, g):
+    """RMS of the network input (f - feq)/w and of the one-step residual
+    target / w over the cells samples come from (whole 3x3 neighbourhood
+    fluid): the scales that make both of order one."""
+    from simuverse_amd import workloads as W
+
+    w = W.W_PHYS[:, None, None]
+    ex, ey = W.EX[:, None, None], W.EY[:, None, None]
+    f = s0.astype(np.float64)
+    rho = f.sum(0)
+    rho[rho == 0] = 1.0
+    ux = ((ex * f).sum(0) + 0.5 * g[0]) / rho
+    uy = ((ey * f).sum(0) + 0.5 * g[1]) / rho
+    eu = ex * ux + ey * uy
+    feq = w * rho * (1 + 3 * eu + 4.5 * eu**2 - 1.5 * (ux**2 + uy**2))
+    om = 1.0 / tau
+    F = (1 - 0.5 * om) * w * ((3 * (ex - ux) + 9 * eu * ex) * g[0] + (3 * (ey - uy) + 9 * eu * ey) * g[1])
+    bgk = f - om * (f - feq) + F
+    nxt = np.stack([np.roll(np.roll(s1[i], -W.EX[i], 1), -W.EY[i], 0) for i in range(9)])
+    near = solid.copy()
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            near |= np.roll(np.roll(solid, dy, 0), dx, 1)
+    fl = ~near
+    return (float(np.sqrt(np.mean(((f - feq) / w)[:, fl] ** 2))),
+            float(np.sqrt(np.mean(((nxt - bgk) / w)[:, fl] ** 2))))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,8 +318,16 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--graph-chunk", type=int, default=100, help="steps per HIP graph (0: direct launches)")
-    ap.add_argument("--workload", choices=["single", "batch"], default="single")
+    ap.add_argument("--workload", choices=["single", "batch", "learned"], default="single")
     ap.add_argument("--f16", action="store_true", help="binary16 deviation storage")
+    ap.add_argument("--learned-force", type=float, default=1e-7, help="learned workload: fine body force")
+    ap.add_argument("--learned-fine-steps", type=int, default=4000)
+    ap.add_argument("--pair-every", type=int, default=100, help="fine steps between training transitions")
+    ap.add_argument("--learned-capacity", type=int, default=4_000_000)
+    ap.add_argument("--learned-train-steps", type=int, default=300)
+    ap.add_argument("--learned-batch", type=int, default=1_000_000)
+    ap.add_argument("--layout", choices=["rows", "planes"], default="rows",
+                    help="device population layout: row-interleaved directions or one plane per direction")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -176,6 +344,9 @@ def main():
     if args.workload == "batch":
         run_batch(args)
         return
+    if args.workload == "learned":
+        run_learned(args)
+        return
 
     n = args.n
     tau, g = 0.55, 1e-6
@@ -183,7 +354,8 @@ def main():
     scene = W.periodic_disks(n, n)
     log(f"scene {n}x{n}: {len(scene.disks)} disks, solid {scene.solid_fraction:.4f} ({time.perf_counter()-t:.1f} s)")
     storage = L.STORAGE_F16_DEV if args.f16 else L.STORAGE_F32
-    lat = L.Lattice(n, n, tau, mode=L.MODE_PHYSICS, storage=storage, force=(g, 0.0))
+    layout = L.LAYOUT_ROWS if args.layout == "rows" else L.LAYOUT_PLANES
+    lat = L.Lattice(n, n, tau, mode=L.MODE_PHYSICS, storage=storage, force=(g, 0.0), layout=layout)
     lat.write_materials(W.materials_from_solid(scene.solid))
     lat.reset()
     lat.set_graph_chunk(args.graph_chunk)
@@ -243,6 +415,7 @@ def main():
             "solid_fraction": round(scene.solid_fraction, 5),
             "disks": int(len(scene.disks)),
             "seed": W.SEED,
+            "layout": args.layout,
             "tile_variant": dict(zip(("block", "cells_per_thread", "vec", "flags"), L.variant_shapes()[variant])),
             "graph_chunk": args.graph_chunk,
             "wall_ms": round(wall * 1e3, 3),

@@ -92,6 +92,13 @@ enum {
 };
 /* y is periodic in both modes; channel walls are solid cells. */
 
+/* Device layout of the physics-mode populations.  Either way the host
+ * exchanges the reference's planar f[i*nx*ny + y*nx + x].
+ *   ROWS    the nine direction rows of each y are adjacent (row pitch
+ *           nx rounded up to 16 elements): element (y*9 + i)*pitch + x
+ *   PLANES  one plane per direction: element i*plane + y*nx + x          */
+enum { LBM_LAYOUT_ROWS = 0, LBM_LAYOUT_PLANES = 1 };
+
 /* ---- materials: the reference's LatticeType (fluid/lattice.rs:15-24) --- */
 enum {
     LBM_MAT_BULK = 1,
@@ -127,7 +134,9 @@ typedef struct lbm_desc {
     float force_y;
     int32_t dev_shift;  /* LBM_STORAGE_F16_DEV: power-of-two scale of the
                            stored deviations (0..14; 0 means the default 10) */
-    int32_t reserved[5];/* must be zero                                   */
+    int32_t layout;     /* physics mode: LBM_LAYOUT_* of the device
+                           populations (parity mode is always planar)     */
+    int32_t reserved[4];/* must be zero                                   */
 } lbm_desc;
 
 /* ---- library ----------------------------------------------------------- */
@@ -300,6 +309,9 @@ int lbm_mlp_set_scales(lbm_mlp* m, float in_scale, float out_scale);
 int lbm_mlp_get_scales(lbm_mlp* m, float* in_scale, float* out_scale);
 /* The raw network on n samples (x: n*9 floats, y: n*9 floats, host). */
 int lbm_mlp_forward(lbm_mlp* m, const float* x, int n, float* y);
+/* The same on the matrix cores: inputs, weights and hidden activations
+ * rounded to bf16, binary32 accumulation (the learned step's default). */
+int lbm_mlp_forward_bf16(lbm_mlp* m, const float* x, int n, float* y);
 /* Loss 1/(9n) sum (y - t)^2 of n host samples and its gradient (5385). */
 int lbm_mlp_loss_grad(lbm_mlp* m, const float* x, const float* t, int n, float* grad, double* loss);
 /* Training samples held on the device. */
@@ -318,9 +330,14 @@ int lbm_mlp_add_transition(lbm_mlp* m, const float* state_t, const float* state_
 int lbm_mlp_train(lbm_mlp* m, int steps, int batch, float lr, uint64_t seed, double* last_loss);
 
 /* Physics-mode handles: steps add the network's projected correction to
- * the BGK collision (binary32 network, parameters staged in LDS); NULL
- * switches back to plain BGK.  The network must outlive its use. */
+ * the BGK collision; NULL switches back to plain BGK.  The network must
+ * outlive its use. */
 int lbm_set_learned_collision(lbm_handle* h, const lbm_mlp* m);
+/* Arithmetic of the learned step's network: LBM_MLP_BF16 (matrix cores,
+ * bf16 operands, binary32 accumulation; default) or LBM_MLP_F32 (binary32
+ * on the vector ALUs). */
+enum { LBM_MLP_BF16 = 0, LBM_MLP_F32 = 1 };
+int lbm_set_learned_precision(lbm_handle* h, int precision);
 /* Largest |u| over fluid cells of the current state and whether every
  * fluid cell is finite with rho > 0. */
 int lbm_health(lbm_handle* h, float* umax, int* finite);

@@ -1,7 +1,9 @@
-"""Builds the gfx950 shared library of the D2Q9 hot path with hipcc.
+"""Builds the gfx950 shared library of the D2Q9 hot path with hipcc, and the
+C++ host layer over its C ABI (D2Q9Node / FluidSimulator) with g++.
 
-The library is written in-tree to ``simuverse_amd/_lib/libsimuverse_lbm.so``
-so that it travels with the source tree to the machine that runs it.
+Both are written in-tree to ``simuverse_amd/_lib/`` (libsimuverse_lbm.so,
+libsimuverse_host.so) so that they travel with the source tree to the
+machine that runs them.
 """
 from __future__ import annotations
 
@@ -16,6 +18,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIBDIR = PKG / "_lib"
 LIB = LIBDIR / "libsimuverse_lbm.so"
+HOST_LIB = LIBDIR / "libsimuverse_host.so"
+HOST_SOURCES = ["host/d2q9_node.cpp"]
 
 SOURCES = ["lbm_kernels.hip", "lbm_api.hip", "lbm_batch_api.hip", "lbm_mlp.hip"]
 
@@ -62,7 +66,21 @@ def build(verbose: bool = False) -> Path:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose: bool = False) -> Path:
+    """The C++ host layer, linked against the HIP library (rpath $ORIGIN)."""
+    tmp = HOST_LIB.with_suffix(".so.tmp")
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", "-I", str(ROOT / "include"),
+           *[str(PKG / s) for s in HOST_SOURCES], "-L", str(LIBDIR), "-lsimuverse_lbm", "-Wl,-rpath,$ORIGIN",
+           "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, HOST_LIB)
+    return HOST_LIB
 
 
 if __name__ == "__main__":

@@ -66,7 +66,8 @@ struct lbm_handle {
     lbm_desc d{};
     int dev = 0;
     hipStream_t stream = nullptr;
-    uint32_t n = 0, plane = 0;
+    uint32_t n = 0, plane = 0, rs = 0;  // population strides (PhysArgs)
+    size_t pop_elems = 0;               // elements per population buffer
     bool f16 = false;
     size_t elem = 4;
     void* buf[2] = {nullptr, nullptr};
@@ -87,18 +88,20 @@ struct lbm_handle {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     const lbm_mlp* mlp = nullptr;   // learned collision correction, if set
+    int mlp_precision = LBM_MLP_BF16;
     uint32_t* health = nullptr;     // [max |u|^2 bits, non-finite flag]
     void* backup = nullptr;         // chunk start state of lbm_run_guarded
 
     bool physics() const { return d.mode == LBM_MODE_PHYSICS; }
     bool force() const { return d.force_x != 0.0f || d.force_y != 0.0f; }
-    size_t pop_bytes() const { return (size_t)9 * plane * elem; }
+    size_t pop_bytes() const { return pop_elems * elem; }
     PhysArgs phys() const
     {
         PhysArgs a;
         a.nx = d.nx;
         a.ny = d.ny;
         a.plane = plane;
+        a.rs = rs;
         a.omega = omega;
         a.fcoef = 1.0f - 0.5f * omega;
         a.gx = d.force_x;
@@ -145,6 +148,10 @@ hipError_t enqueue_step(lbm_handle* h, int swap, bool export_macro)
         if (export_macro) return hipErrorNotSupported;
         float s_in, s_out;
         lbm_mlp_scales(h->mlp, &s_in, &s_out);
+        if (h->mlp_precision == LBM_MLP_BF16)
+            return launch_phys_step_learned_mfma(h->f16, h->d.x_edge, h->force(), h->phys(), src, dst, h->bits,
+                                                 h->words, lbm_mlp_device_packed(h->mlp),
+                                                 lbm_mlp_device_params(h->mlp), s_in, s_out, h->stream);
         return launch_phys_step_learned(h->f16, h->d.x_edge, h->force(), h->phys(), src, dst, h->bits, h->words,
                                         lbm_mlp_device_params(h->mlp), s_in, s_out, h->stream);
     }
@@ -241,8 +248,11 @@ int validate_desc(const lbm_desc* d)
     if (!d) return fail(LBM_ERR_INVALID, "null desc");
     if (d->nx < 2 || d->ny < 2) return fail(LBM_ERR_INVALID, "lattice %dx%d: both sides must be >= 2", d->nx, d->ny);
     const uint64_t n = (uint64_t)d->nx * (uint64_t)d->ny;
-    if (n * 9 + 9 * 64 > 0xFFFFFFFFull)
+    const uint64_t pitch = ((uint64_t)d->nx + 15u) & ~15ull;
+    if (n * 9 + 9 * 64 > 0xFFFFFFFFull || pitch * (uint64_t)d->ny * 9 > 0xFFFFFFFFull)
         return fail(LBM_ERR_INVALID, "lattice %dx%d exceeds 32-bit population indexing", d->nx, d->ny);
+    if (d->layout != LBM_LAYOUT_ROWS && d->layout != LBM_LAYOUT_PLANES)
+        return fail(LBM_ERR_INVALID, "unknown layout %d", d->layout);
     if (!(d->tau > 0.0f) || !std::isfinite(d->tau)) return fail(LBM_ERR_INVALID, "tau must be finite and > 0");
     if (d->mode != LBM_MODE_PARITY_WGSL && d->mode != LBM_MODE_PHYSICS)
         return fail(LBM_ERR_INVALID, "unknown mode %d", d->mode);
@@ -333,7 +343,16 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out)
     h->d = *desc;
     h->dev = dev;
     h->n = (uint32_t)((uint64_t)desc->nx * (uint64_t)desc->ny);
-    h->plane = (h->n + 63u) & ~63u;
+    if (desc->mode == LBM_MODE_PHYSICS && desc->layout == LBM_LAYOUT_ROWS) {
+        const uint32_t pitch = ((uint32_t)desc->nx + 15u) & ~15u;
+        h->plane = pitch;
+        h->rs = 9u * pitch;
+        h->pop_elems = (size_t)h->rs * (size_t)desc->ny;
+    } else {
+        h->plane = (h->n + 63u) & ~63u;
+        h->rs = (uint32_t)desc->nx;
+        h->pop_elems = (size_t)9 * h->plane;
+    }
     h->f16 = desc->storage == LBM_STORAGE_F16_DEV;
     h->elem = h->f16 ? 2 : 4;
     h->words = solid_mask_words(desc->nx);
@@ -466,7 +485,7 @@ int lbm_write_populations(lbm_handle* h, int buf, const float* f)
         float* stage = nullptr;
         HIP_TRY(hipMalloc(&stage, 9 * nb));
         hipError_t e = hipMemcpyAsync(stage, f, 9 * nb, hipMemcpyHostToDevice, h->stream);
-        if (e == hipSuccess) e = launch_host_to_dev(h->f16, stage, h->buf[buf], h->n, h->plane, h->scale, h->stream);
+        if (e == hipSuccess) e = launch_host_to_dev(h->f16, h->phys(), stage, h->buf[buf], h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         (void)hipFree(stage);
         if (e != hipSuccess) return fail(LBM_ERR_HIP, "population upload: %s", hipGetErrorString(e));
@@ -490,7 +509,7 @@ int lbm_read_populations(lbm_handle* h, int buf, float* f)
     } else {
         float* stage = nullptr;
         HIP_TRY(hipMalloc(&stage, 9 * nb));
-        hipError_t e = launch_dev_to_host(h->f16, h->buf[buf], stage, h->n, h->plane, h->inv_scale, h->stream);
+        hipError_t e = launch_dev_to_host(h->f16, h->phys(), h->buf[buf], stage, h->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(f, stage, 9 * nb, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         (void)hipFree(stage);
@@ -759,6 +778,17 @@ int lbm_set_learned_collision(lbm_handle* h, const lbm_mlp* m)
     if (m && !h->physics()) return lbm_fail(LBM_ERR_UNSUPPORTED, "the learned correction needs physics mode");
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->mlp = m;
+    drop_graphs(h);
+    return LBM_OK;
+}
+
+int lbm_set_learned_precision(lbm_handle* h, int precision)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (precision != LBM_MLP_BF16 && precision != LBM_MLP_F32)
+        return lbm_fail(LBM_ERR_INVALID, "unknown network precision %d", precision);
+    h->mlp_precision = precision;
     drop_graphs(h);
     return LBM_OK;
 }

@@ -203,7 +203,8 @@ int lbm_batch_create(const lbm_domain_desc* domains, int count, int storage, lbm
         for (int r : q.reserved)
             if (r) return lbm_fail(LBM_ERR_INVALID, "domain %d: reserved fields must be zero", d);
         const uint64_t n = (uint64_t)q.nx * (uint64_t)q.ny;
-        elems += 9 * ((n + 63) & ~63ull);
+        const uint64_t pitch = ((uint64_t)q.nx + 15u) & ~15ull;
+        elems += (9 * pitch * (uint64_t)q.ny + 63u) & ~63ull;
         cells += n;
         words += (uint64_t)solid_mask_words(q.nx) * (uint64_t)q.ny;
     }
@@ -233,7 +234,10 @@ int lbm_batch_create(const lbm_domain_desc* domains, int count, int storage, lbm
         const uint32_t n = (uint32_t)q.nx * (uint32_t)q.ny;
         D.a.nx = q.nx;
         D.a.ny = q.ny;
-        D.a.plane = (n + 63u) & ~63u;
+        // row-interleaved populations, row pitch nx rounded up to 16
+        const uint32_t pitch = ((uint32_t)q.nx + 15u) & ~15u;
+        D.a.plane = pitch;
+        D.a.rs = 9u * pitch;
         D.a.omega = 1.0f / q.tau;
         D.a.fcoef = 1.0f - 0.5f * D.a.omega;
         D.a.gx = q.force_x;
@@ -249,7 +253,7 @@ int lbm_batch_create(const lbm_domain_desc* domains, int count, int storage, lbm
         D.force = (q.force_x != 0.0f || q.force_y != 0.0f) ? 1 : 0;
         b->dom.push_back(D);
         b->cell_off.push_back((size_t)cell);
-        pop += 9ull * D.a.plane;
+        pop += ((uint64_t)D.a.rs * (uint64_t)q.ny + 63u) & ~63ull;
         cell += n;
         word += (uint64_t)D.words * (uint64_t)q.ny;
         b->max_cells = std::max(b->max_cells, n);
@@ -355,8 +359,7 @@ int lbm_batch_write_populations(lbm_batch* b, int d, const float* f)
     BATCH_TRY(hipMalloc(&stage, 9 * nb));
     hipError_t e = hipMemcpyAsync(stage, f, 9 * nb, hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess)
-        e = launch_host_to_dev(b->f16, stage, b->view(b->cur, d), b->dom[d].ncells, b->dom[d].a.plane, b->scale,
-                               b->stream);
+        e = launch_host_to_dev(b->f16, b->dom[d].a, stage, b->view(b->cur, d), b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     (void)hipFree(stage);
     if (e != hipSuccess) return lbm_fail(LBM_ERR_HIP, "population upload: %s", hipGetErrorString(e));
@@ -374,8 +377,7 @@ int lbm_batch_read_populations(lbm_batch* b, int d, float* f)
     const size_t nb = (size_t)b->dom[d].ncells * sizeof(float);
     float* stage = nullptr;
     BATCH_TRY(hipMalloc(&stage, 9 * nb));
-    hipError_t e = launch_dev_to_host(b->f16, b->view(b->cur, d), stage, b->dom[d].ncells, b->dom[d].a.plane,
-                                      b->inv_scale, b->stream);
+    hipError_t e = launch_dev_to_host(b->f16, b->dom[d].a, b->view(b->cur, d), stage, b->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(f, stage, 9 * nb, hipMemcpyDeviceToHost, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     (void)hipFree(stage);

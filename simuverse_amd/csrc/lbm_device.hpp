@@ -118,10 +118,16 @@ struct Store<_Float16> {
     __device__ static inline float scale_out(float g, float scale) { return g * scale; }
 };
 
-// Arguments shared by the physics-mode kernels of one domain.
+// Arguments shared by the physics-mode kernels of one domain.  Population i
+// of cell (x, y) is element y*rs + x + i*plane of a buffer:
+//   row-interleaved  plane = pitch, rs = 9*pitch (the nine rows of one y are
+//                    adjacent, so the 18 streams of a step walk 3 + 1
+//                    contiguous regions of HBM instead of 18 far-apart ones)
+//   planar           plane >= nx*ny, rs = nx
 struct PhysArgs {
     int32_t nx, ny;
-    uint32_t plane;   // elements between direction planes (>= nx*ny)
+    uint32_t plane;   // elements between directions
+    uint32_t rs;      // elements between rows
     float omega;      // 1/tau
     float fcoef;      // Guo prefactor 1 - omega/2
     float gx, gy;     // body force
@@ -129,6 +135,11 @@ struct PhysArgs {
     float scale;      // binary16 deviation scale 2^shift
     float inv_scale;  // 2^-shift
 };
+
+__host__ __device__ inline uint32_t pop_index(const PhysArgs& a, int x, int y, int i)
+{
+    return (uint32_t)y * a.rs + (uint32_t)x + (uint32_t)i * a.plane;
+}
 
 // One domain of a batch: its physics arguments and where its populations,
 // solid mask and chunks start.
@@ -190,6 +201,146 @@ __device__ inline void mlp_forward_one(const float* __restrict__ P, const float 
         const float h2 = gelu(s);
 #pragma unroll
         for (int m = 0; m < kMlpOut; ++m) y[m] += P[kOffW3 + m * kMlpH + j] * h2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The network on matrix cores: bf16 operands, binary32 accumulation.
+//
+// One wavefront evaluates 64 samples as two tiles of 32 with
+// v_mfma_f32_32x32x16_bf16 on the transposed problem H = W X (M = units,
+// N = samples, K = inputs).  Operand layouts of that instruction, lane l,
+// h = l / 32, element j:
+//   A (M x K)   row l % 32, k = 8 h + j            (j < 8)
+//   B (K x N)   column l % 32, k = 8 h + j          (j < 8)
+//   C (M x N)   column l % 32, row 8 (j / 4) + 4 h + (j % 4)   (j < 16)
+// A layer's C tile therefore already sits in the lanes of the samples it
+// belongs to; its 16 values per lane become the next layer's B operands for
+// two K steps once the K axis of the next weight matrix is permuted to match
+// (chain_unit).  No LDS round trip between layers.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Hidden unit that K index k (0..63) of a chained layer stands for: K step
+// s = k / 16 reads C values j = 8 (s % 2) + k % 8 of M tile s / 2 from lane
+// half h = (k / 8) % 2.
+__host__ __device__ constexpr int chain_unit(int k)
+{
+    return 32 * ((k >> 4) >> 1) + 8 * ((8 * ((k >> 4) & 1) + (k & 7)) >> 2) + 4 * ((k >> 3) & 1) +
+           ((8 * ((k >> 4) & 1) + (k & 7)) & 3);
+}
+// Row of C value j in lane half h of an M tile.
+__host__ __device__ constexpr int c_row(int j, int h) { return 8 * (j >> 2) + 4 * h + (j & 3); }
+
+// Lane-ordered bf16 weights (8 per lane per operand, 16 B, read by one
+// ds_read_b128):
+//   W1p[mt][lane][8]      mt < 2, from W1 (K = inputs padded to 16)
+//   W2p[mt][s][lane][8]   mt < 2, s < 4, from W2 with K permuted by chain_unit
+//   W3p[s][lane][8]       s < 4, rows = outputs padded to 32
+constexpr int kPackW1 = 0;
+constexpr int kPackW2 = kPackW1 + 2 * 64 * 8;
+constexpr int kPackW3 = kPackW2 + 2 * 4 * 64 * 8;
+constexpr int kPackElems = kPackW3 + 4 * 64 * 8;  // 7168 bf16 = 14 KiB
+
+// Packed weight element e (0 <= e < kPackElems) from the binary32 parameter
+// vector P.
+__host__ __device__ inline float pack_weight(const float* P, int e)
+{
+    if (e < kPackW2) {
+        const int mt = e / 512, lane = (e / 8) % 64, j = e % 8;
+        const int in = 8 * (lane >> 5) + j;
+        return in < kMlpIn ? P[kOffW1 + (32 * mt + (lane & 31)) * kMlpIn + in] : 0.0f;
+    }
+    if (e < kPackW3) {
+        const int r = e - kPackW2;
+        const int mt = r / 2048, s = (r / 512) % 4, lane = (r / 8) % 64, j = r % 8;
+        const int k = 16 * s + 8 * (lane >> 5) + j;
+        return P[kOffW2 + (32 * mt + (lane & 31)) * kMlpH + chain_unit(k)];
+    }
+    const int r = e - kPackW3;
+    const int s = r / 512, lane = (r / 8) % 64, j = r % 8;
+    const int o = lane & 31, k = 16 * s + 8 * (lane >> 5) + j;
+    return o < kMlpOut ? P[kOffW3 + o * kMlpH + chain_unit(k)] : 0.0f;
+}
+
+// GELU with erf from Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7, far
+// below bf16 resolution), on the hardware reciprocal and exp2.
+__device__ __forceinline__ float gelu_fast(float x)
+{
+    const float z = fabsf(x) * 0.70710678118654752f;
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, z, 1.0f));
+    const float poly =
+        t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f),
+                 0.254829592f);
+    const float e = __builtin_amdgcn_exp2f(-(z * z) * 1.44269504088896341f);
+    const float erf_abs = 1.0f - poly * e;
+    return 0.5f * x * (1.0f + copysignf(erf_abs, x));
+}
+
+__device__ __forceinline__ bf16x8 lds_b128(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+// Network on the 64 samples of one wavefront.  W: packed weights (LDS);
+// bias: b1[64], b2[64], b3[9] in binary32 (LDS); xs: 64 x 16 bf16 inputs
+// (row per sample, inputs 9..15 zero); ys: 64 x 9 binary32 outputs.  Every
+// lane of the wavefront must call this (matrix instructions use all lanes).
+__device__ __forceinline__ void mlp_wave_bf16(const __bf16* __restrict__ W, const float* __restrict__ bias,
+                                              const __bf16* __restrict__ xs, float* __restrict__ ys, int lane)
+{
+    const int h = lane >> 5, n = lane & 31;
+    const float* b1 = bias;
+    const float* b2 = bias + kMlpH;
+    const float* b3 = bias + 2 * kMlpH;
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        const bf16x8 xb = lds_b128(xs + (32 * nt + n) * 16 + 8 * h);
+        bf16x8 hb[4];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            f32x16 c;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) c[j] = b1[32 * mt + c_row(j, h)];
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b128(W + kPackW1 + (mt * 64 + lane) * 8), xb, c, 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) hb[2 * mt + (j >> 3)][j & 7] = (__bf16)gelu_fast(c[j]);
+        }
+        bf16x8 hb2[4];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            f32x16 c;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) c[j] = b2[32 * mt + c_row(j, h)];
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b128(W + kPackW2 + ((mt * 4 + s) * 64 + lane) * 8),
+                                                            hb[s], c, 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) hb2[2 * mt + (j >> 3)][j & 7] = (__bf16)gelu_fast(c[j]);
+        }
+        f32x16 c;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int o = c_row(j, h);
+            c[j] = o < kMlpOut ? b3[o] : 0.0f;
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b128(W + kPackW3 + (s * 64 + lane) * 8), hb2[s], c, 0, 0,
+                                                        0);
+        // Outputs 0..8 are rows 0-3 and 8 (lane half 0) and 4-7 (half 1).
+        float* y = ys + (32 * nt + n) * kMlpOut;
+        if (h == 0) {
+            y[0] = c[0];
+            y[1] = c[1];
+            y[2] = c[2];
+            y[3] = c[3];
+            y[8] = c[4];
+        } else {
+            y[4] = c[0];
+            y[5] = c[1];
+            y[6] = c[2];
+            y[7] = c[3];
+        }
     }
 }
 

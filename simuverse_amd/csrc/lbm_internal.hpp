@@ -42,12 +42,11 @@ hipError_t launch_wgsl_reset(const WgslArgs& a, int fluid_ty, float* A, float* B
 hipError_t launch_wgsl_reduce(const WgslArgs& a, const float* src, const uint8_t* mat, double* partial,
                               int nblocks, double* out, hipStream_t s);
 
-// Host-format populations (dense binary32 f) <-> physics-mode device planes
-// of deviations (binary32, or binary16 scaled by 2^shift).
-hipError_t launch_host_to_dev(bool f16, const float* in, void* out, uint32_t n, uint32_t plane, float scale,
-                              hipStream_t s);
-hipError_t launch_dev_to_host(bool f16, const void* in, float* out, uint32_t n, uint32_t plane, float inv_scale,
-                              hipStream_t s);
+// Host-format populations (dense binary32 f[i*nx*ny + y*nx + x]) <->
+// physics-mode device storage of deviations (binary32, or binary16 scaled
+// by 2^shift) in the layout a.plane / a.rs.
+hipError_t launch_host_to_dev(bool f16, const PhysArgs& a, const float* in, void* out, hipStream_t s);
+hipError_t launch_dev_to_host(bool f16, const PhysArgs& a, const void* in, float* out, hipStream_t s);
 hipError_t launch_pack_rgba16f(const float* macro, uint32_t n, uint16_t* out, hipStream_t s);
 hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s);
 
@@ -67,6 +66,9 @@ hipError_t launch_batch_step(int variant, bool f16, const BatchDomainDev* doms, 
 hipError_t launch_phys_step_learned(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, void* dst,
                                    const uint32_t* bits, uint32_t words, const float* params, float s_in,
                                    float s_out, hipStream_t s);
+hipError_t launch_phys_step_learned_mfma(bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
+                                        void* dst, const uint32_t* bits, uint32_t words, const __bf16* packed,
+                                        const float* params, float s_in, float s_out, hipStream_t s);
 hipError_t launch_restrict4(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, const uint8_t* mat,
                             float nscale, float gcx, float gcy, float* fc, uint8_t* solid_c, float* uxc, float* uyc,
                             hipStream_t s);
@@ -78,6 +80,8 @@ hipError_t launch_phys_health(bool f16, int xedge, bool force, const PhysArgs& a
 struct lbm_mlp;
 // Device pointer to a network's binary32 parameters, and its scales.
 const float* lbm_mlp_device_params(const lbm_mlp* m);
+// Device pointer to its lane-ordered bf16 weights (kPackElems).
+const __bf16* lbm_mlp_device_packed(const lbm_mlp* m);
 void lbm_mlp_scales(const lbm_mlp* m, float* s_in, float* s_out);
 
 namespace lbm {

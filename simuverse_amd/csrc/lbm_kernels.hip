@@ -43,8 +43,7 @@ __device__ inline void phys_gather(const PhysArgs& a, const T* __restrict__ src,
         if (xm < 0) xm = nx - 1;
         if (xp >= nx) xp = 0;
     }
-    const uint32_t rm = (uint32_t)ym * (uint32_t)nx, r0 = (uint32_t)y * (uint32_t)nx,
-                   rp = (uint32_t)yp * (uint32_t)nx;
+    const uint32_t rm = (uint32_t)ym * a.rs, r0 = (uint32_t)y * a.rs, rp = (uint32_t)yp * a.rs;
     const uint32_t cc = r0 + (uint32_t)cx;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -125,6 +124,110 @@ __device__ inline uint32_t solid3(const uint32_t* __restrict__ bits, uint32_t wo
     return (uint32_t)(v >> ((uint32_t)x & 31u)) & 7u;
 }
 
+// Neighbourhood of a cell: the wrapped rows above and below, and the solid
+// flags (x-1, x, x+1) of rows ym, y, yp from three 64-bit windows of the
+// packed mask.
+struct CellNbr {
+    int ym, yp;
+    uint32_t sm, s0, sp;
+};
+
+__device__ __forceinline__ CellNbr cell_nbr(const PhysArgs& a, const uint32_t* __restrict__ bits, uint32_t words,
+                                            int x, int y)
+{
+    CellNbr nb;
+    nb.ym = (y == 0) ? a.ny - 1 : y - 1;
+    nb.yp = (y == a.ny - 1) ? 0 : y + 1;
+    nb.sm = solid3(bits, words, nb.ym, x);
+    nb.s0 = solid3(bits, words, y, x);
+    nb.sp = solid3(bits, words, nb.yp, x);
+    return nb;
+}
+
+// Post-streaming deviations of the fluid cell (x, y) (see phys_gather; the
+// outlet test uses the west bit of the mask).
+template <typename T, int XEDGE, int FLAGS>
+__device__ __forceinline__ void cell_gather(const PhysArgs& a, const T* __restrict__ src, const CellNbr& nb, int x,
+                                            int y, float g[9])
+{
+    const int nx = a.nx;
+    int cx = x;
+    if (XEDGE == 1 && x == nx - 1 && !(nb.s0 & 1u)) cx = nx - 2;
+    int xm = cx - 1, xp = cx + 1;
+    if (XEDGE == 0) {
+        if (xm < 0) xm = nx - 1;
+        if (xp >= nx) xp = 0;
+    }
+    const uint32_t r0 = (uint32_t)y * a.rs, rm = (uint32_t)nb.ym * a.rs, rp = (uint32_t)nb.yp * a.rs;
+    const uint32_t cc = r0 + (uint32_t)cx;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int ex = ex_of(i), ey = ey_of(i);
+        const int sx = ex == 1 ? xm : (ex == -1 ? xp : cx);
+        const uint32_t srow = ey == 1 ? rm : (ey == -1 ? rp : r0);
+        bool oob = false;
+        if (XEDGE == 1 && ex != 0) oob = (sx < 0) || (sx >= nx);
+        const uint32_t k = oob ? cc + (uint32_t)inv_of(i) * a.plane : srow + (uint32_t)sx + (uint32_t)i * a.plane;
+        g[i] = Store<T>::from_storage(ld<(FLAGS & kNtLoad) != 0>(src + k), a.inv_scale);
+    }
+    if (XEDGE == 1 && x == 0) {
+        const float u0 = a.inlet_ux;
+        const float G = g[0] + g[2] + g[4] + 2.0f * (g[3] + g[6] + g[7]);
+        const float rho = (1.0f + G) / (1.0f - u0);
+        const float ru = rho * u0;
+        const float h = 0.5f * (g[4] - g[2]);
+        g[1] = g[3] + (2.0f / 3.0f) * ru;
+        g[5] = g[7] + h + (1.0f / 6.0f) * ru;
+        g[8] = g[6] - h + (1.0f / 6.0f) * ru;
+    }
+}
+
+// Stores the post-collision deviations of the fluid cell (x, y) in its own
+// slots, and pushes f_i into slot inv(i) of each solid neighbour x + e_i
+// (the mask's edge bits are empty for a non-periodic x, so nothing is pushed
+// outside).
+template <typename T, int XEDGE, int FLAGS>
+__device__ __forceinline__ void cell_store(const PhysArgs& a, T* __restrict__ dst, const CellNbr& nb, int x, int y,
+                                           const float g[9])
+{
+    const int nx = a.nx;
+    const uint32_t r0 = (uint32_t)y * a.rs;
+    const uint32_t pc = r0 + (uint32_t)x;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+        st<(FLAGS & kNtStore) != 0>(dst + pc + (uint32_t)i * a.plane, Store<T>::to_storage(g[i], a.scale));
+    if ((nb.sm | nb.sp | (nb.s0 & 5u)) == 0u) return;
+    int xm = x - 1, xp = x + 1;
+    if (XEDGE == 0) {
+        if (xm < 0) xm = nx - 1;
+        if (xp >= nx) xp = 0;
+    }
+    const uint32_t rm = (uint32_t)nb.ym * a.rs, rp = (uint32_t)nb.yp * a.rs;
+#pragma unroll
+    for (int i = 1; i < 9; ++i) {
+        const int ex = ex_of(i), ey = ey_of(i);
+        const uint32_t srow = ey == 1 ? nb.sp : (ey == -1 ? nb.sm : nb.s0);
+        if (!((srow >> (1 + ex)) & 1u)) continue;
+        const int tx = ex == 1 ? xp : (ex == -1 ? xm : x);
+        const uint32_t trow = ey == 1 ? rp : (ey == -1 ? rm : r0);
+        Store<T>::store_dev(dst, trow + (uint32_t)tx + (uint32_t)inv_of(i) * a.plane, g[i], w_phys(i), a.scale);
+    }
+}
+
+// The learned correction's input: the non-equilibrium part g - geq = f - feq
+// over w_i s_in.
+__device__ __forceinline__ void cell_features(const float g[9], float rho, float ux, float uy, float s_in,
+                                              float xin[9])
+{
+    const float dr = rho - 1.0f, usq = 1.5f * (ux * ux + uy * uy);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float eu = (float)ex_of(i) * ux + (float)ey_of(i) * uy;
+        const float geq = w_phys(i) * (dr + rho * (3.0f * eu + 4.5f * (eu * eu) - usq));
+        xin[i] = (g[i] - geq) / (w_phys(i) * s_in);
+    }
+}
+
 // LEARNED adds the network's projected correction to the collision; mlp
 // then points at its parameters (in LDS) and s_in, s_out are its scales.
 template <typename T, int XEDGE, bool FORCE, bool EXPORT, int FLAGS = 0, bool LEARNED = false>
@@ -133,16 +236,9 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
                                  float* __restrict__ macro, uint32_t n, int x, int y,
                                  const float* __restrict__ mlp = nullptr, float s_in = 1.0f, float s_out = 1.0f)
 {
-    const int nx = a.nx, ny = a.ny;
-    const uint32_t r0 = (uint32_t)y * (uint32_t)nx;
-    const uint32_t c = r0 + (uint32_t)x;
-    const int ym = (y == 0) ? ny - 1 : y - 1;
-    const int yp = (y == ny - 1) ? 0 : y + 1;
-    // Solid flags of the 3x3 neighbourhood: three 64-bit windows of the mask.
-    const uint32_t sm = solid3(bits, words, ym, x);
-    const uint32_t s0 = solid3(bits, words, y, x);
-    const uint32_t sp = solid3(bits, words, yp, x);
-    if (s0 & 2u) {
+    const uint32_t c = (uint32_t)y * (uint32_t)a.nx + (uint32_t)x;  // cell (macro planes)
+    const CellNbr nb = cell_nbr(a, bits, words, x, y);
+    if (nb.s0 & 2u) {
         if (EXPORT) {
             macro[c] = 0.0f;
             macro[c + n] = 0.0f;
@@ -152,38 +248,7 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
         return;
     }
     float g[9];
-    {
-        // Gather (see phys_gather); the outlet test uses the west bit.
-        int cx = x;
-        if (XEDGE == 1 && x == nx - 1 && !(s0 & 1u)) cx = nx - 2;
-        int xm = cx - 1, xp = cx + 1;
-        if (XEDGE == 0) {
-            if (xm < 0) xm = nx - 1;
-            if (xp >= nx) xp = 0;
-        }
-        const uint32_t rm = (uint32_t)ym * (uint32_t)nx, rp = (uint32_t)yp * (uint32_t)nx;
-        const uint32_t cc = r0 + (uint32_t)cx;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            const int ex = ex_of(i), ey = ey_of(i);
-            const int sx = ex == 1 ? xm : (ex == -1 ? xp : cx);
-            const uint32_t srow = ey == 1 ? rm : (ey == -1 ? rp : r0);
-            bool oob = false;
-            if (XEDGE == 1 && ex != 0) oob = (sx < 0) || (sx >= nx);
-            const uint32_t k = oob ? cc + (uint32_t)inv_of(i) * a.plane : srow + (uint32_t)sx + (uint32_t)i * a.plane;
-            g[i] = Store<T>::from_storage(ld<(FLAGS & kNtLoad) != 0>(src + k), a.inv_scale);
-        }
-        if (XEDGE == 1 && x == 0) {
-            const float u0 = a.inlet_ux;
-            const float G = g[0] + g[2] + g[4] + 2.0f * (g[3] + g[6] + g[7]);
-            const float rho = (1.0f + G) / (1.0f - u0);
-            const float ru = rho * u0;
-            const float h = 0.5f * (g[4] - g[2]);
-            g[1] = g[3] + (2.0f / 3.0f) * ru;
-            g[5] = g[7] + h + (1.0f / 6.0f) * ru;
-            g[8] = g[6] - h + (1.0f / 6.0f) * ru;
-        }
-    }
+    cell_gather<T, XEDGE, FLAGS>(a, src, nb, x, y, g);
     float rho, ux, uy;
     phys_moments<FORCE>(a, g, rho, ux, uy);
     if (EXPORT) {
@@ -194,15 +259,8 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
     }
     float corr[9];
     if constexpr (LEARNED) {
-        // network input: non-equilibrium part (g - geq = f - feq) over w s_in
-        const float dr = rho - 1.0f, usq = 1.5f * (ux * ux + uy * uy);
         float xin[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            const float eu = (float)ex_of(i) * ux + (float)ey_of(i) * uy;
-            const float geq = w_phys(i) * (dr + rho * (3.0f * eu + 4.5f * (eu * eu) - usq));
-            xin[i] = (g[i] - geq) / (w_phys(i) * s_in);
-        }
+        cell_features(g, rho, ux, uy, s_in, xin);
         mlp_forward_one(mlp, xin, corr);
 #pragma unroll
         for (int i = 0; i < 9; ++i) corr[i] *= w_phys(i) * s_out;
@@ -213,28 +271,7 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
 #pragma unroll
         for (int i = 0; i < 9; ++i) g[i] += corr[i];
     }
-
-#pragma unroll
-    for (int i = 0; i < 9; ++i)
-        st<(FLAGS & kNtStore) != 0>(dst + c + (uint32_t)i * a.plane, Store<T>::to_storage(g[i], a.scale));
-    // Pushes into solid neighbours x + e_i (slot inv(i)); the mask's edge
-    // bits are empty for a non-periodic x, so nothing is pushed outside.
-    if ((sm | sp | (s0 & 5u)) == 0u) return;
-    int xm = x - 1, xp = x + 1;
-    if (XEDGE == 0) {
-        if (xm < 0) xm = nx - 1;
-        if (xp >= nx) xp = 0;
-    }
-    const uint32_t rm = (uint32_t)ym * (uint32_t)nx, rp = (uint32_t)yp * (uint32_t)nx;
-#pragma unroll
-    for (int i = 1; i < 9; ++i) {
-        const int ex = ex_of(i), ey = ey_of(i);
-        const uint32_t srow = ey == 1 ? sp : (ey == -1 ? sm : s0);
-        if (!((srow >> (1 + ex)) & 1u)) continue;
-        const int tx = ex == 1 ? xp : (ex == -1 ? xm : x);
-        const uint32_t trow = ey == 1 ? rp : (ey == -1 ? rm : r0);
-        Store<T>::store_dev(dst, trow + (uint32_t)tx + (uint32_t)inv_of(i) * a.plane, g[i], w_phys(i), a.scale);
-    }
+    cell_store<T, XEDGE, FLAGS>(a, dst, nb, x, y, g);
 }
 
 // One step of the whole lattice.  Block = BS threads on a row segment of
@@ -258,8 +295,8 @@ __global__ __launch_bounds__(BS) void k_phys_step(PhysArgs a, const T* __restric
 }
 
 // Four consecutive cells of a row per lane.  Loads move 4 elements per
-// plane (16 B in binary32; 4 B-aligned for the x-shifted planes), stores
-// are aligned.  Only lanes whose 4 cells and their x-neighbours lie inside
+// direction (16 B in binary32; 4 B-aligned for the x-shifted rows), stores
+// are aligned (the row stride and direction stride are multiples of 4).  Only lanes whose 4 cells and their x-neighbours lie inside
 // the row (1 <= x0, x0 + 4 <= nx - 1) take this path; the per-cell
 // arithmetic is phys_moments/phys_collide, as in phys_cell, so results are
 // identical bit for bit.
@@ -318,13 +355,13 @@ __device__ inline void phys_cells4(const PhysArgs& a, const T* __restrict__ src,
     const int nx = a.nx, ny = a.ny;
     const int ym = (y == 0) ? ny - 1 : y - 1;
     const int yp = (y == ny - 1) ? 0 : y + 1;
-    const uint32_t r0 = (uint32_t)y * (uint32_t)nx, rm = (uint32_t)ym * (uint32_t)nx,
-                   rp = (uint32_t)yp * (uint32_t)nx;
+    const uint32_t r0 = (uint32_t)y * a.rs, rm = (uint32_t)ym * a.rs, rp = (uint32_t)yp * a.rs;
     const uint32_t sm = solid6(bits, words, ym, x0);
     const uint32_t s0 = solid6(bits, words, y, x0);
     const uint32_t sp = solid6(bits, words, yp, x0);
     const uint32_t self = (s0 >> 1) & 15u;  // bit k: cell x0 + k is solid
-    const uint32_t c0 = r0 + (uint32_t)x0;
+    const uint32_t c0 = (uint32_t)y * (uint32_t)nx + (uint32_t)x0;  // cell (macro planes)
+    const uint32_t p0 = r0 + (uint32_t)x0;                         // population element
     if (self == 15u) {
         if (EXPORT)
 #pragma unroll
@@ -364,7 +401,7 @@ __device__ inline void phys_cells4(const PhysArgs& a, const T* __restrict__ src,
             float o[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) o[k] = Store<T>::scale_out(g[i][k], a.scale);
-            Vec4<T>::store(dst + c0 + (uint32_t)i * a.plane, o);
+            Vec4<T>::store(dst + p0 + (uint32_t)i * a.plane, o);
         }
     } else {
         // A solid cell's slots receive pushes from its neighbours during
@@ -374,7 +411,7 @@ __device__ inline void phys_cells4(const PhysArgs& a, const T* __restrict__ src,
             if ((self >> k) & 1u) continue;
 #pragma unroll
             for (int i = 0; i < 9; ++i)
-                Store<T>::store_dev(dst, c0 + (uint32_t)k + (uint32_t)i * a.plane, g[i][k], w_phys(i), a.scale);
+                Store<T>::store_dev(dst, p0 + (uint32_t)k + (uint32_t)i * a.plane, g[i][k], w_phys(i), a.scale);
         }
     }
     if ((sm | sp | s0) == 0u) return;
@@ -582,6 +619,77 @@ __global__ __launch_bounds__(256) void k_phys_step_learned(PhysArgs a, const T* 
                                                       (uint32_t)a.nx * (uint32_t)a.ny, x, y, P, s_in, s_out);
 }
 
+// One step with the learned correction evaluated on the matrix cores
+// (bf16 operands, binary32 accumulation; mlp_wave_bf16).  Blocks of 4
+// wavefronts loop over row segments of 256 cells, one cell per lane, so the
+// 14 KiB of packed weights are staged in LDS once per block.  Every lane
+// takes part in the matrix instructions; lanes beyond the row or on solid
+// cells feed zeros and store nothing.
+template <typename T, int XEDGE, bool FORCE>
+__global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, const T* __restrict__ src,
+                                                                T* __restrict__ dst, const uint32_t* __restrict__ bits,
+                                                                uint32_t words, const __bf16* __restrict__ packed,
+                                                                const float* __restrict__ params, float s_in,
+                                                                float s_out, uint32_t nseg)
+{
+    __shared__ __align__(16) __bf16 W[kPackElems];
+    __shared__ float bias[2 * kMlpH + 16];
+    __shared__ __align__(16) __bf16 xs[4][64 * 16];
+    __shared__ float ys[4][64 * kMlpOut];
+    {
+        const uint4* from = reinterpret_cast<const uint4*>(packed);
+        uint4* to = reinterpret_cast<uint4*>(W);
+        for (int k = threadIdx.x; k < kPackElems / 8; k += 256) to[k] = from[k];
+        for (int k = threadIdx.x; k < kMlpH; k += 256) {
+            bias[k] = params[kOffB1 + k];
+            bias[kMlpH + k] = params[kOffB2 + k];
+        }
+        if (threadIdx.x < kMlpOut) bias[2 * kMlpH + threadIdx.x] = params[kOffB3 + threadIdx.x];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t total = nseg * (uint32_t)a.ny;
+    for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+        const int y = (int)(tile / nseg);
+        const int x = (int)(tile % nseg) * 256 + (int)threadIdx.x;
+        bool fluid = false;
+        CellNbr nb{};
+        float g[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        float rho = 1.0f, ux = 0.0f, uy = 0.0f;
+        if (x < a.nx) {
+            nb = cell_nbr(a, bits, words, x, y);
+            fluid = !(nb.s0 & 2u);
+        }
+        float xin[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (fluid) {
+            cell_gather<T, XEDGE, kNtStore>(a, src, nb, x, y, g);
+            phys_moments<FORCE>(a, g, rho, ux, uy);
+            cell_features(g, rho, ux, uy, s_in, xin);
+        }
+        bf16x8 lo, hi;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            lo[i] = (__bf16)xin[i];
+            hi[i] = (__bf16)(i == 0 ? xin[8] : 0.0f);
+        }
+        *reinterpret_cast<bf16x8*>(&xs[wave][lane * 16]) = lo;
+        *reinterpret_cast<bf16x8*>(&xs[wave][lane * 16 + 8]) = hi;
+        __syncthreads();
+        mlp_wave_bf16(W, bias, xs[wave], ys[wave], lane);
+        __syncthreads();
+        if (fluid) {
+            float corr[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) corr[i] = ys[wave][lane * kMlpOut + i] * (w_phys(i) * s_out);
+            project_correction(corr);
+            phys_collide<FORCE>(a, g, rho, ux, uy);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) g[i] += corr[i];
+            cell_store<T, XEDGE, kNtStore>(a, dst, nb, x, y, g);
+        }
+    }
+}
+
 // Builds the packed solid mask from the material bytes (one 32-bit word per
 // thread).
 __global__ __launch_bounds__(256) void k_build_solid_bits(const uint8_t* __restrict__ mat, int nx, int ny,
@@ -650,7 +758,7 @@ __global__ __launch_bounds__(256) void k_phys_prime(PhysArgs a, T* __restrict__ 
             continue;
         }
         const uint32_t t = (uint32_t)ty * (uint32_t)nx + (uint32_t)tx;
-        if (mat_solid(mat[t])) buf[t + (uint32_t)inv_of(i) * a.plane] = buf[c + (uint32_t)i * a.plane];
+        if (mat_solid(mat[t])) buf[pop_index(a, tx, ty, inv_of(i))] = buf[pop_index(a, x, y, i)];
     }
 }
 
@@ -663,10 +771,11 @@ __global__ __launch_bounds__(256) void k_phys_fill(PhysArgs a, T* __restrict__ b
     const uint32_t c = blockIdx.x * 256u + threadIdx.x;
     if (c >= n) return;
     const bool s = mat_solid(mat[c]) != 0;
+    const uint32_t p = pop_index(a, (int)(c % (uint32_t)a.nx), (int)(c / (uint32_t)a.nx), 0);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        if (s) buf[c + (uint32_t)i * a.plane] = (T)0;
-        else Store<T>::store_dev(buf, c + (uint32_t)i * a.plane, 0.0f, w_phys(i), a.scale);
+        if (s) buf[p + (uint32_t)i * a.plane] = (T)0;
+        else Store<T>::store_dev(buf, p + (uint32_t)i * a.plane, 0.0f, w_phys(i), a.scale);
     }
 }
 
@@ -893,7 +1002,7 @@ __global__ __launch_bounds__(256) void k_phys_reduce(PhysArgs a, const T* __rest
         const int y = (int)(c / (uint32_t)nx), x = (int)(c % (uint32_t)nx);
         int cx = x;
         if (XEDGE == 1 && x == nx - 1 && !mat_solid(mat[c - 1])) cx = nx - 2;
-        const uint32_t cc = (uint32_t)y * (uint32_t)nx + (uint32_t)cx;
+        const uint32_t cc = pop_index(a, cx, y, 0);
         double f[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
@@ -909,8 +1018,7 @@ __global__ __launch_bounds__(256) void k_phys_reduce(PhysArgs a, const T* __rest
             }
             if (oob) f[i] = Store<T>::load_f64(src, cc + (uint32_t)inv_of(i) * a.plane, w_phys(i), a.inv_scale);
             else
-                f[i] = Store<T>::load_f64(src, (uint32_t)sy * (uint32_t)nx + (uint32_t)sx + (uint32_t)i * a.plane,
-                                          w_phys(i), a.inv_scale);
+                f[i] = Store<T>::load_f64(src, pop_index(a, sx, sy, i), w_phys(i), a.inv_scale);
         }
         if (XEDGE == 1 && x == 0) {
             const double u0 = (double)a.inlet_ux;
@@ -938,7 +1046,7 @@ __global__ __launch_bounds__(256) void k_phys_reduce(PhysArgs a, const T* __rest
             }
             const uint32_t t = (uint32_t)ty * (uint32_t)nx + (uint32_t)tx;
             if (!mat_solid(mat[t])) continue;
-            const double v = Store<T>::load_f64(src, t + (uint32_t)inv_of(i) * a.plane, w_phys(i), a.inv_scale);
+            const double v = Store<T>::load_f64(src, pop_index(a, tx, ty, inv_of(i)), w_phys(i), a.inv_scale);
             acc[3] += 2.0 * (double)ex_of(i) * v;
             acc[4] += 2.0 * (double)ey_of(i) * v;
         }
@@ -1008,27 +1116,29 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__
 // Format conversion and export.
 // ---------------------------------------------------------------------------
 
-// Dense host layout f[i*n + c] (binary32) <-> device planes of deviations.
+// Dense host layout f[i*n + c] (binary32) <-> device storage of deviations.
 template <typename T>
-__global__ __launch_bounds__(256) void k_host_to_dev(const float* __restrict__ in, T* __restrict__ out, uint32_t n,
-                                                     uint32_t plane, float scale)
+__global__ __launch_bounds__(256) void k_host_to_dev(PhysArgs a, const float* __restrict__ in, T* __restrict__ out)
 {
+    const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
     const uint32_t c = blockIdx.x * 256u + threadIdx.x;
     if (c >= n) return;
+    const uint32_t p = pop_index(a, (int)(c % (uint32_t)a.nx), (int)(c / (uint32_t)a.nx), 0);
 #pragma unroll
     for (int i = 0; i < 9; ++i)
-        out[c + (uint32_t)i * plane] = Store<T>::to_storage(in[c + (uint32_t)i * n] - w_phys(i), scale);
+        out[p + (uint32_t)i * a.plane] = Store<T>::to_storage(in[c + (uint32_t)i * n] - w_phys(i), a.scale);
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_dev_to_host(const T* __restrict__ in, float* __restrict__ out, uint32_t n,
-                                                     uint32_t plane, float inv_scale)
+__global__ __launch_bounds__(256) void k_dev_to_host(PhysArgs a, const T* __restrict__ in, float* __restrict__ out)
 {
+    const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
     const uint32_t c = blockIdx.x * 256u + threadIdx.x;
     if (c >= n) return;
+    const uint32_t p = pop_index(a, (int)(c % (uint32_t)a.nx), (int)(c / (uint32_t)a.nx), 0);
 #pragma unroll
     for (int i = 0; i < 9; ++i)
-        out[c + (uint32_t)i * n] = w_phys(i) + Store<T>::from_storage(in[c + (uint32_t)i * plane], inv_scale);
+        out[c + (uint32_t)i * n] = w_phys(i) + Store<T>::from_storage(in[p + (uint32_t)i * a.plane], a.inv_scale);
 }
 
 // (ux, uy, rho, w) planes -> rgba16float texels, round to nearest even.
@@ -1251,6 +1361,27 @@ hipError_t launch_phys_step_learned(bool f16, int xedge, bool force, const PhysA
     return hipGetLastError();
 }
 
+hipError_t launch_phys_step_learned_mfma(bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
+                                        void* dst, const uint32_t* bits, uint32_t words, const __bf16* packed,
+                                        const float* params, float s_in, float s_out, hipStream_t s)
+{
+    const uint32_t nseg = (uint32_t)((a.nx + 255) / 256);
+    const uint32_t tiles = nseg * (uint32_t)a.ny;
+    const dim3 grid(tiles < 1024u ? tiles : 1024u), block(256);
+#define LBM_L(T, XE, FO)                                                                                       \
+    hipLaunchKernelGGL((k_phys_step_learned_mfma<T, XE, FO>), grid, block, 0, s, a, (const T*)src, (T*)dst, bits, \
+                       words, packed, params, s_in, s_out, nseg)
+    if (f16) {
+        if (xedge) { if (force) LBM_L(_Float16, 1, true); else LBM_L(_Float16, 1, false); }
+        else { if (force) LBM_L(_Float16, 0, true); else LBM_L(_Float16, 0, false); }
+    } else {
+        if (xedge) { if (force) LBM_L(float, 1, true); else LBM_L(float, 1, false); }
+        else { if (force) LBM_L(float, 0, true); else LBM_L(float, 0, false); }
+    }
+#undef LBM_L
+    return hipGetLastError();
+}
+
 hipError_t launch_build_solid_bits(const uint8_t* mat, int nx, int ny, int periodic_x, uint32_t words,
                                   uint32_t* bits, hipStream_t s)
 {
@@ -1352,27 +1483,23 @@ hipError_t launch_wgsl_reduce(const WgslArgs& a, const float* src, const uint8_t
     return hipGetLastError();
 }
 
-hipError_t launch_host_to_dev(bool f16, const float* in, void* out, uint32_t n, uint32_t plane, float scale,
-                              hipStream_t s)
+hipError_t launch_host_to_dev(bool f16, const PhysArgs& a, const float* in, void* out, hipStream_t s)
 {
+    const unsigned nb = blocks_for((size_t)a.nx * a.ny, 256);
     if (f16)
-        hipLaunchKernelGGL(k_host_to_dev<_Float16>, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, (_Float16*)out, n,
-                           plane, scale);
+        hipLaunchKernelGGL(k_host_to_dev<_Float16>, dim3(nb), dim3(256), 0, s, a, in, (_Float16*)out);
     else
-        hipLaunchKernelGGL(k_host_to_dev<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, (float*)out, n, plane,
-                           scale);
+        hipLaunchKernelGGL(k_host_to_dev<float>, dim3(nb), dim3(256), 0, s, a, in, (float*)out);
     return hipGetLastError();
 }
 
-hipError_t launch_dev_to_host(bool f16, const void* in, float* out, uint32_t n, uint32_t plane, float inv_scale,
-                              hipStream_t s)
+hipError_t launch_dev_to_host(bool f16, const PhysArgs& a, const void* in, float* out, hipStream_t s)
 {
+    const unsigned nb = blocks_for((size_t)a.nx * a.ny, 256);
     if (f16)
-        hipLaunchKernelGGL(k_dev_to_host<_Float16>, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const _Float16*)in,
-                           out, n, plane, inv_scale);
+        hipLaunchKernelGGL(k_dev_to_host<_Float16>, dim3(nb), dim3(256), 0, s, a, (const _Float16*)in, out);
     else
-        hipLaunchKernelGGL(k_dev_to_host<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const float*)in, out, n,
-                           plane, inv_scale);
+        hipLaunchKernelGGL(k_dev_to_host<float>, dim3(nb), dim3(256), 0, s, a, (const float*)in, out);
     return hipGetLastError();
 }
 

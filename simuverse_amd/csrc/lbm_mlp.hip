@@ -57,6 +57,53 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ p
 }
 
 // ---------------------------------------------------------------------------
+// Matrix-core forward (bf16 operands, binary32 accumulation): lane-ordered
+// bf16 weights are rebuilt from the binary32 parameters whenever those
+// change; a block of 4 wavefronts takes 256 samples.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mlp_pack_bf16(const float* __restrict__ params, __bf16* __restrict__ packed)
+{
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e < kPackElems) packed[e] = (__bf16)pack_weight(params, e);
+}
+
+// Stages the packed weights and the biases b1, b2, b3 of a network in LDS.
+__device__ inline void stage_bf16_net(const __bf16* __restrict__ packed, const float* __restrict__ params,
+                                      __bf16* W, float* bias)
+{
+    const uint4* src = reinterpret_cast<const uint4*>(packed);
+    uint4* dst = reinterpret_cast<uint4*>(W);
+    for (int k = threadIdx.x; k < kPackElems / 8; k += blockDim.x) dst[k] = src[k];
+    for (int k = threadIdx.x; k < kMlpH; k += blockDim.x) {
+        bias[k] = params[kOffB1 + k];
+        bias[kMlpH + k] = params[kOffB2 + k];
+    }
+    for (int k = threadIdx.x; k < kMlpOut; k += blockDim.x) bias[2 * kMlpH + k] = params[kOffB3 + k];
+}
+
+__global__ __launch_bounds__(256) void k_mlp_forward_bf16(const __bf16* __restrict__ packed,
+                                                          const float* __restrict__ params,
+                                                          const float* __restrict__ X, float* __restrict__ Y, int n)
+{
+    __shared__ __align__(16) __bf16 W[kPackElems];
+    __shared__ float bias[2 * kMlpH + 16];
+    __shared__ __align__(16) __bf16 xs[4][64 * 16];
+    __shared__ float ys[4][64 * kMlpOut];
+    stage_bf16_net(packed, params, W, bias);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s = blockIdx.x * 256 + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        xs[wave][lane * 16 + i] = (__bf16)((s < n && i < kMlpIn) ? X[(size_t)s * kMlpIn + i] : 0.0f);
+    __syncthreads();
+    mlp_wave_bf16(W, bias, xs[wave], ys[wave], lane);
+    __syncthreads();
+    if (s >= n) return;
+#pragma unroll
+    for (int m = 0; m < kMlpOut; ++m) Y[(size_t)s * kMlpOut + m] = ys[wave][lane * kMlpOut + m];
+}
+
+// ---------------------------------------------------------------------------
 // Loss gradient: a block of 256 threads takes 64 samples; thread (s, q)
 // = (tid % 64, tid / 64) computes hidden units 16q..16q+15 of sample s.
 // Activations are staged in LDS (rows padded to 65 floats), then the block
@@ -277,6 +324,7 @@ struct lbm_mlp {
     int dev = 0;
     hipStream_t stream = nullptr;
     float* params = nullptr;  // binary32 master copy
+    __bf16* packed = nullptr; // lane-ordered bf16 weights of the matrix-core path
     float* adam_m = nullptr;
     float* adam_v = nullptr;
     float* grad = nullptr;
@@ -304,7 +352,7 @@ int mlp_check(lbm_mlp* m)
 
 void mlp_free(lbm_mlp* m)
 {
-    for (void* p : {(void*)m->params, (void*)m->adam_m, (void*)m->adam_v, (void*)m->grad, (void*)m->partial,
+    for (void* p : {(void*)m->params, (void*)m->packed, (void*)m->adam_m, (void*)m->adam_v, (void*)m->grad, (void*)m->partial,
                     (void*)m->loss_part, (void*)m->loss, (void*)m->X, (void*)m->T, (void*)m->count})
         if (p) (void)hipFree(p);
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -334,7 +382,18 @@ int mlp_grad(lbm_mlp* m, const float* X, const float* T, int n, int batch, uint6
     return LBM_OK;
 }
 
+// Rebuilds the bf16 weights after the binary32 parameters changed.
+int mlp_repack(lbm_mlp* m)
+{
+    hipLaunchKernelGGL(k_mlp_pack_bf16, dim3((kPackElems + 255) / 256), dim3(256), 0, m->stream, m->params,
+                       m->packed);
+    MLP_TRY(hipGetLastError());
+    return LBM_OK;
+}
+
 }  // namespace
+
+const __bf16* lbm_mlp_device_packed(const lbm_mlp* m) { return m->packed; }
 
 // Device pointer to the binary32 parameters and the two scales of a
 // network (used by the learned collision step).
@@ -359,6 +418,7 @@ int lbm_mlp_create(uint64_t seed, int capacity, lbm_mlp** out)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
     const size_t pb = kMlpParams * sizeof(float);
     if (e == hipSuccess) e = hipMalloc(&m->params, pb);
+    if (e == hipSuccess) e = hipMalloc(&m->packed, kPackElems * sizeof(__bf16));
     if (e == hipSuccess) e = hipMalloc(&m->adam_m, pb);
     if (e == hipSuccess) e = hipMalloc(&m->adam_v, pb);
     if (e == hipSuccess) e = hipMalloc(&m->grad, pb);
@@ -392,9 +452,9 @@ int lbm_mlp_create(uint64_t seed, int capacity, lbm_mlp** out)
     e = hipMemcpy(m->params, p.data(), pb, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(m->adam_m, 0, pb);
     if (e == hipSuccess) e = hipMemset(m->adam_v, 0, pb);
-    if (e != hipSuccess) {
+    if (e != hipSuccess || mlp_repack(m) != LBM_OK || hipStreamSynchronize(m->stream) != hipSuccess) {
         mlp_free(m);
-        return lbm_fail(LBM_ERR_HIP, "network init: %s", hipGetErrorString(e));
+        return lbm_fail(LBM_ERR_HIP, "network init failed");
     }
     *out = m;
     return LBM_OK;
@@ -417,6 +477,7 @@ int lbm_mlp_set_params(lbm_mlp* m, const float* params)
     MLP_TRY(hipMemcpyAsync(m->params, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, m->stream));
     MLP_TRY(hipMemsetAsync(m->adam_m, 0, kMlpParams * sizeof(float), m->stream));
     MLP_TRY(hipMemsetAsync(m->adam_v, 0, kMlpParams * sizeof(float), m->stream));
+    if ((rc = mlp_repack(m))) return rc;
     MLP_TRY(hipStreamSynchronize(m->stream));
     m->adam_step = 0;
     return LBM_OK;
@@ -450,7 +511,7 @@ int lbm_mlp_get_scales(lbm_mlp* m, float* in_scale, float* out_scale)
     return LBM_OK;
 }
 
-int lbm_mlp_forward(lbm_mlp* m, const float* x, int n, float* y)
+static int mlp_forward_impl(lbm_mlp* m, const float* x, int n, float* y, bool bf16)
 {
     int rc = mlp_check(m);
     if (rc) return rc;
@@ -461,7 +522,11 @@ int lbm_mlp_forward(lbm_mlp* m, const float* x, int n, float* y)
     hipError_t e = hipMalloc(&dy, (size_t)n * kMlpOut * sizeof(float));
     if (e == hipSuccess) e = hipMemcpyAsync(dx, x, (size_t)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, m->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_mlp_forward, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->params, dx, dy, n);
+        if (bf16)
+            hipLaunchKernelGGL(k_mlp_forward_bf16, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->packed,
+                               m->params, dx, dy, n);
+        else
+            hipLaunchKernelGGL(k_mlp_forward, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->params, dx, dy, n);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(y, dy, (size_t)n * kMlpOut * sizeof(float), hipMemcpyDeviceToHost, m->stream);
@@ -471,6 +536,10 @@ int lbm_mlp_forward(lbm_mlp* m, const float* x, int n, float* y)
     if (e != hipSuccess) return lbm_fail(LBM_ERR_HIP, "network forward: %s", hipGetErrorString(e));
     return LBM_OK;
 }
+
+int lbm_mlp_forward(lbm_mlp* m, const float* x, int n, float* y) { return mlp_forward_impl(m, x, n, y, false); }
+
+int lbm_mlp_forward_bf16(lbm_mlp* m, const float* x, int n, float* y) { return mlp_forward_impl(m, x, n, y, true); }
 
 int lbm_mlp_loss_grad(lbm_mlp* m, const float* x, const float* t, int n, float* grad, double* loss)
 {
@@ -580,6 +649,7 @@ int lbm_mlp_train(lbm_mlp* m, int steps, int batch, float lr, uint64_t seed, dou
                            m->adam_m, m->adam_v, lr, b1, b2, eps, c1, c2);
         MLP_TRY(hipGetLastError());
     }
+    if ((rc = mlp_repack(m))) return rc;
     if (last_loss) MLP_TRY(hipMemcpyAsync(last_loss, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->stream));
     MLP_TRY(hipStreamSynchronize(m->stream));
     return LBM_OK;

@@ -25,6 +25,10 @@ STORAGE_F32 = 0
 STORAGE_F16_DEV = 1
 X_PERIODIC = 0
 X_INLET_OUTLET = 1
+LAYOUT_ROWS = 0
+LAYOUT_PLANES = 1
+MLP_BF16 = 0
+MLP_F32 = 1
 MAT_BULK, MAT_BOUNDARY, MAT_INLET, MAT_OBSTACLE, MAT_OUTLET, MAT_EXTERNAL_FORCE, MAT_GHOST = 1, 2, 3, 4, 5, 6, 7
 
 LATTICE_INFO_DTYPE = np.dtype([("material", "<i4"), ("block_iter", "<i4"), ("vx", "<f4"), ("vy", "<f4")])
@@ -49,7 +53,8 @@ class LbmDesc(C.Structure):
         ("force_x", C.c_float),
         ("force_y", C.c_float),
         ("dev_shift", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("layout", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -116,6 +121,7 @@ _SIGNATURES = [
     ("lbm_mlp_set_scales", C.c_int, [_P, C.c_float, C.c_float]),
     ("lbm_mlp_get_scales", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("lbm_mlp_forward", C.c_int, [_P, _P, C.c_int, _P]),
+    ("lbm_mlp_forward_bf16", C.c_int, [_P, _P, C.c_int, _P]),
     ("lbm_mlp_loss_grad", C.c_int, [_P, _P, _P, C.c_int, _P, C.POINTER(C.c_double)]),
     ("lbm_mlp_clear_samples", C.c_int, [_P]),
     ("lbm_mlp_sample_count", C.c_int, [_P, C.POINTER(C.c_int)]),
@@ -124,6 +130,7 @@ _SIGNATURES = [
                                          C.c_int, C.c_int, C.POINTER(C.c_int)]),
     ("lbm_mlp_train", C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_uint64, C.POINTER(C.c_double)]),
     ("lbm_set_learned_collision", C.c_int, [_P, _P]),
+    ("lbm_set_learned_precision", C.c_int, [_P, C.c_int]),
     ("lbm_health", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_int)]),
     ("lbm_run_guarded", C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_int)]),
     ("lbm_restrict4", C.c_int, [_P, C.c_float, C.c_float, C.c_float, _P, _P, _P, _P]),
@@ -217,6 +224,7 @@ class Lattice:
         inlet_ux: float = 0.0,
         force: tuple[float, float] = (0.0, 0.0),
         dev_shift: int = 0,
+        layout: int = LAYOUT_ROWS,
     ):
         self._lib = load_library()
         self.nx, self.ny = int(nx), int(ny)
@@ -226,6 +234,7 @@ class Lattice:
         d.nx, d.ny, d.tau = self.nx, self.ny, tau
         d.mode, d.storage, d.fluid_ty, d.x_edge = mode, storage, fluid_ty, x_edge
         d.inlet_ux, d.force_x, d.force_y, d.dev_shift = inlet_ux, force[0], force[1], dev_shift
+        d.layout = layout
         self.desc = d
         h = C.c_void_p()
         _check(self._lib.lbm_create(C.byref(d), C.byref(h)))
@@ -329,6 +338,10 @@ class Lattice:
     def set_learned_collision(self, net: "Network | None") -> None:
         _check(self._lib.lbm_set_learned_collision(self._h, net.handle if net is not None else None))
         self._net = net  # keep the network alive while attached
+
+    def set_learned_precision(self, precision: int) -> None:
+        """MLP_BF16 (matrix cores, the default) or MLP_F32 (vector ALUs)."""
+        _check(self._lib.lbm_set_learned_precision(self._h, precision))
 
     def health(self) -> tuple[float, bool]:
         u, f = C.c_float(), C.c_int()
@@ -503,10 +516,11 @@ class Network:
         _check(self._lib.lbm_mlp_get_scales(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
-    def forward(self, x: np.ndarray) -> np.ndarray:
+    def forward(self, x: np.ndarray, bf16: bool = False) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 9)
         y = np.empty_like(x)
-        _check(self._lib.lbm_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(y)))
+        fn = self._lib.lbm_mlp_forward_bf16 if bf16 else self._lib.lbm_mlp_forward
+        _check(fn(self._h, _ptr(x), x.shape[0], _ptr(y)))
         return y
 
     def loss_grad(self, x: np.ndarray, t: np.ndarray) -> tuple[float, np.ndarray]:

@@ -280,3 +280,28 @@ def batch_domains(count: int = 512, seed: int = SEED, size_min: int = 64, size_m
             solid |= (x - cx) ** 2 + (y - cy) ** 2 <= r * r
         out.append(BatchDomain(nx, ny, tau, ux, solid, np.array(disks, np.int64).reshape(-1, 3)))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Learned-collision workload: a periodic cylinder array, fine and 4x coarse
+# ---------------------------------------------------------------------------
+
+
+def cylinder_array_solid(n: int, per_side: int = 4, radius: int = 128) -> np.ndarray:
+    """n x n periodic lattice with per_side^2 cylinders of `radius` centred
+    in equal square cells of the array: solid iff (x-cx)^2 + (y-cy)^2 <= r^2."""
+    pitch = n // per_side
+    y, x = np.mgrid[0:pitch, 0:pitch]
+    c = pitch // 2
+    tile = (x - c) ** 2 + (y - c) ** 2 <= radius * radius
+    return np.tile(tile, (per_side, per_side))
+
+
+def equilibrium_state(nx: int, ny: int, ux: float, uy: float = 0.0, solid: np.ndarray | None = None) -> np.ndarray:
+    """f = feq(rho = 1, u) on fluid cells (binary64), 0 on solid cells."""
+    eu = EX * ux + EY * uy
+    feq = W_PHYS * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * (ux * ux + uy * uy))
+    f = np.broadcast_to(feq[:, None, None], (9, ny, nx)).copy()
+    if solid is not None:
+        f[:, solid] = 0.0
+    return f

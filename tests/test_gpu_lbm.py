@@ -167,12 +167,12 @@ def test_parity_reference_scene_with_reset_and_edits():
 
 
 def physics_pair(nx, ny, tau, solid, f0, steps, x_edge=0, inlet_ux=0.0, force=(0.0, 0.0),
-                 storage=L.STORAGE_F32):
+                 storage=L.STORAGE_F32, layout=L.LAYOUT_ROWS):
     p = O.phys_params(nx, ny, tau, x_edge=x_edge, inlet_ux=inlet_ux, force=force)
     st = O.PhysState(p, solid, f0)
     st.run(steps)
     lat = L.Lattice(nx, ny, tau, mode=L.MODE_PHYSICS, storage=storage, x_edge=x_edge, inlet_ux=inlet_ux,
-                    force=force)
+                    force=force, layout=layout)
     lat.write_materials(W.materials_from_solid(solid))
     lat.write_populations(f0.astype(np.float32), 0)
     lat.run(steps)
@@ -194,7 +194,11 @@ def macro_errors(st, lat, solid):
     }
 
 
-def test_physics_c1_channel_cylinder_vs_fp64():
+LAYOUTS = pytest.mark.parametrize("layout", [L.LAYOUT_ROWS, L.LAYOUT_PLANES], ids=["rows", "planes"])
+
+
+@LAYOUTS
+def test_physics_c1_channel_cylinder_vs_fp64(layout):
     """256x256 channel: Zou-He inlet ux=0.05, zero-gradient outlet, walls,
     cylinder r=16 at (64,128), tau=0.6, f0 = w + U(-1e-4,1e-4), 1000 steps.
     binary32 storage and arithmetic against binary64: rho and u relative L2
@@ -203,21 +207,22 @@ def test_physics_c1_channel_cylinder_vs_fp64():
     nx = ny = 256
     solid = W.cylinder_channel_solid()
     f0 = W.rest_state(nx, ny, solid, amplitude=1e-4)
-    st, lat = physics_pair(nx, ny, 0.6, solid, f0, 1000, x_edge=1, inlet_ux=0.05)
+    st, lat = physics_pair(nx, ny, 0.6, solid, f0, 1000, x_edge=1, inlet_ux=0.05, layout=layout)
     e = macro_errors(st, lat, solid)
     print("physics C1 errors vs fp64:", e, flush=True)
     assert e["rho_l2"] < 1e-5 and e["u_l2"] < 1e-5
     assert e["u_max"] < 1e-5 * 10
 
 
-def test_physics_periodic_disks_force_vs_fp64():
+@LAYOUTS
+def test_physics_periodic_disks_force_vs_fp64(layout):
     """The roofline scene at 256^2: periodic, Guo force g=1e-6 (x), seeded
     disks r in 4..24 on >= 5 % of the cells, tau = 0.55, 1000 steps."""
     print("case: physics periodic disks 256x256 1000 steps", flush=True)
     nx = ny = 256
     scene = W.periodic_disks(nx, ny)
     f0 = W.rest_state(nx, ny, scene.solid)
-    st, lat = physics_pair(nx, ny, 0.55, scene.solid, f0, 1000, force=(1e-6, 0.0))
+    st, lat = physics_pair(nx, ny, 0.55, scene.solid, f0, 1000, force=(1e-6, 0.0), layout=layout)
     e = macro_errors(st, lat, scene.solid)
     print("physics periodic-disk errors vs fp64:", e, flush=True)
     # The force adds 1e-7 per step to deviations of ~1e-3, which binary32
@@ -254,13 +259,19 @@ def test_physics_reduce_identities_small():
 
 
 def test_tile_variants_and_graphs_bit_identical():
+    """Every tile variant, with and without graphs, in both device layouts
+    (row pitch 304 > nx = 300 in the row-interleaved one) gives the same
+    populations bit for bit."""
     print("case: variants + graphs 300x200", flush=True)
     nx, ny = 300, 200
     scene = W.periodic_disks(nx, ny, 0.05, 4, 10, seed=4)
     f0 = W.rest_state(nx, ny, scene.solid, amplitude=1e-3).astype(np.float32)
     outs = []
-    for v in range(len(L.variant_shapes())):
-        lat = L.Lattice(nx, ny, 0.58, force=(1e-5, 0.0))
+    nv = len(L.variant_shapes())
+    for v in range(2 * nv):
+        layout = L.LAYOUT_ROWS if v < nv else L.LAYOUT_PLANES
+        v %= nv
+        lat = L.Lattice(nx, ny, 0.58, force=(1e-5, 0.0), layout=layout)
         lat.write_materials(W.materials_from_solid(scene.solid))
         lat.write_populations(f0, 0)
         lat.set_variant(v)

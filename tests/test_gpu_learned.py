@@ -1,7 +1,9 @@
 """Learned collision correction on the GPU, through the C ABI.
 
-* the network's binary32 forward and loss gradient against the binary64
+* the network's binary32 forward and loss gradient, and its matrix-core
+  (bf16 operands, binary32 accumulation) forward, against the binary64
   oracle (oracle/mlp_oracle.c);
+* the matrix-core learned step tracks the binary32 one;
 * training (device forward/backward/Adam) reduces the loss;
 * the projected correction leaves mass and momentum exactly as plain BGK
   does;
@@ -59,6 +61,33 @@ def test_gradient_matches_fp64_oracle():
         assert np.linalg.norm(g[a:b] - gref[a:b]) <= 1e-4 * np.linalg.norm(gref[a:b]) + 1e-12
 
 
+def test_forward_bf16_matrix_cores_vs_fp64_oracle():
+    """Matrix-core forward: inputs, weights and hidden activations rounded
+    to bf16 (8 significant bits, relative rounding <= 2^-9), binary32
+    accumulation.  A few such roundings per layer give an output error of
+    a few 1e-3 of its scale: bounded here by 1e-2 (rms) and 3e-2 (max).  A
+    wrong operand layout or unit permutation gives O(1) errors."""
+    print("case: network forward bf16 (matrix cores) vs fp64", flush=True)
+    net = L.Network(seed=3)
+    p = random_params(1, 0.15)
+    net.set_params(p)
+    x = np.random.default_rng(2).standard_normal((1000, 9)).astype(np.float32)
+    y = net.forward(x, bf16=True)
+    ref = O.mlp_forward(p.astype(np.float64), x.astype(np.float64))
+    err = y - ref
+    rms = float(np.sqrt(np.mean(ref**2)))
+    print(f"bf16 forward: rms err {np.sqrt(np.mean(err**2)) / rms:.3e}, max err {np.max(np.abs(err)) / np.max(np.abs(ref)):.3e}",
+          flush=True)
+    assert np.sqrt(np.mean(err**2)) < 1e-2 * rms
+    assert np.max(np.abs(err)) < 3e-2 * np.max(np.abs(ref))
+    # every output column carries signal and matches on its own
+    for m in range(9):
+        assert np.linalg.norm(err[:, m]) < 2e-2 * np.linalg.norm(ref[:, m])
+    # bf16 parameters follow set_params
+    net.set_params(np.zeros(5385, np.float32))
+    assert np.array_equal(net.forward(x, bf16=True), np.zeros_like(x))
+
+
 def test_training_reduces_loss():
     print("case: network training", flush=True)
     rng = np.random.default_rng(6)
@@ -74,8 +103,9 @@ def test_training_reduces_loss():
     assert l1 < 0.1 * l0
 
 
-def test_correction_conserves_mass_and_momentum():
-    print("case: learned correction conserves", flush=True)
+@pytest.mark.parametrize("precision", [L.MLP_BF16, L.MLP_F32], ids=["bf16", "f32"])
+def test_correction_conserves_mass_and_momentum(precision):
+    print(f"case: learned correction conserves ({precision})", flush=True)
     nx, ny = 96, 64
     net = L.Network(seed=5)
     net.set_params(random_params(7, 0.2))
@@ -84,6 +114,7 @@ def test_correction_conserves_mass_and_momentum():
     lat = L.Lattice(nx, ny, 0.6)
     lat.write_populations(f0, 0)
     lat.set_learned_collision(net)
+    lat.set_learned_precision(precision)
     r0 = lat.reduce()
     lat.run(50)
     r1 = lat.reduce()
@@ -94,6 +125,35 @@ def test_correction_conserves_mass_and_momentum():
     plain.write_populations(f0, 0)
     plain.run(50)
     assert not np.array_equal(lat.read_populations(), plain.read_populations())
+
+
+def test_learned_step_bf16_tracks_f32():
+    """The matrix-core learned step against the binary32 one on a lattice
+    whose width (160) leaves part of a 256-cell segment empty, with disks
+    and a body force: after 50 steps the two differ by a few percent of what
+    the correction itself changed."""
+    print("case: learned step bf16 vs f32", flush=True)
+    nx, ny = 160, 96
+    scene = W.periodic_disks(nx, ny, 0.05, 4, 8, seed=5)
+    f0 = W.rest_state(nx, ny, scene.solid, amplitude=1e-3, seed=4).astype(np.float32)
+    net = L.Network(seed=6)
+    net.set_params(random_params(10, 0.2))
+    net.set_scales(1e-3, 1e-4)
+    out = {}
+    for name, prec in (("plain", None), ("f32", L.MLP_F32), ("bf16", L.MLP_BF16)):
+        lat = L.Lattice(nx, ny, 0.6, force=(1e-5, 0.0))
+        lat.write_materials(W.materials_from_solid(scene.solid))
+        lat.write_populations(f0, 0)
+        if prec is not None:
+            lat.set_learned_collision(net)
+            lat.set_learned_precision(prec)
+        lat.run(50)
+        out[name] = lat.read_populations()[:, ~scene.solid].astype(np.float64)
+    d_corr = np.linalg.norm(out["f32"] - out["plain"])
+    d_bf16 = np.linalg.norm(out["bf16"] - out["f32"])
+    print(f"correction effect {d_corr:.3e}, bf16 vs f32 {d_bf16:.3e}", flush=True)
+    assert d_corr > 0
+    assert d_bf16 < 5e-2 * d_corr
 
 
 def test_blow_up_falls_back_to_bgk():
@@ -147,13 +207,25 @@ def test_restriction_keeps_block_mass_and_momentum():
     np.testing.assert_allclose(fc.sum(axis=0)[cf], (m / nfl)[cf], rtol=2e-6)
     np.testing.assert_allclose(uxc[cf], (blk(np.where(fl, rho * ux, 0.0)) / m)[cf], rtol=1e-4, atol=1e-9)
     jx = sum(W.EX[i] * fc[i] for i in range(9))
-    np.testing.assert_allclose((jx + 2 * g[0]) / fc.sum(axis=0), uxc, rtol=1e-4, atol=1e-8)
+    # the coarse Guo velocity (sum e f + g_c/2)/rho with g_c = 4 g equals u_c
+    np.testing.assert_allclose(((jx + 2 * g[0]) / fc.sum(axis=0))[cf], uxc[cf], rtol=1e-4, atol=1e-8)
+    assert np.all(fc[:, sc] == 0) and np.all(uxc[sc] == 0)
+
+
+def interior(solid):
+    """Cells whose whole 3x3 neighbourhood (periodic) is fluid: the cells
+    lbm_mlp_add_transition takes samples from."""
+    near = solid.copy()
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            near |= np.roll(np.roll(solid, dy, 0), dx, 1)
+    return ~near
 
 
 def rms_scales(s0, s1, solid, tau, g):
     """Root-mean-square of the network input (f - feq)/w and of the
-    residual target / w over the fluid cells of one transition: the
-    normalisations that make both of order one."""
+    residual target / w over the cells samples come from (3x3 fluid) of one
+    transition: the normalisations that make both of order one."""
     w = W.W_PHYS[:, None, None]
     ex, ey = W.EX[:, None, None], W.EY[:, None, None]
     f = s0.astype(np.float64)
@@ -166,7 +238,7 @@ def rms_scales(s0, s1, solid, tau, g):
     F = (1 - 0.5 * om) * w * ((3 * (ex - ux) + 9 * eu * ex) * g[0] + (3 * (ey - uy) + 9 * eu * ey) * g[1])
     bgk = f - om * (f - feq) + F
     nxt = np.stack([np.roll(np.roll(s1[i], -W.EX[i], 1), -W.EY[i], 0) for i in range(9)])
-    fl = ~solid
+    fl = interior(solid)
     return float(np.sqrt(np.mean(((f - feq) / w)[:, fl] ** 2))), float(np.sqrt(np.mean(((nxt - bgk) / w)[:, fl] ** 2)))
 
 
