@@ -214,6 +214,12 @@ int lbm_read_macro(lbm_handle* h, float* rho, float* ux, float* uy);
 /* The same field packed like macro_tex: rgba16float (ux, uy, rho, w) per
  * texel, w = 1 on fluid and 0 on solid cells, round-to-nearest-even. */
 int lbm_read_macro_rgba16f(lbm_handle* h, uint16_t* rgba);
+/* The reference's curl pass (assets/wgsl/lbm/curl_update.wgsl:12-34, whose
+ * dispatch fluid_simulator.rs:226,230 leaves commented out) on that macro
+ * field: rgba16float texels (curl * 3.5 + 0.5, 0, 0, 0) with
+ * curl = uy(x+1) - uy(x-1) + ux(y-1) - ux(y+1) of the binary16 texels,
+ * neighbours clamped at 0; past the last column / row the texel is zero. */
+int lbm_read_curl_rgba16f(lbm_handle* h, uint16_t* rgba);
 /* Moments of the current state (the populations the next step gathers):
  * the values the next step would export. */
 int lbm_macro(lbm_handle* h, float* rho, float* ux, float* uy);

@@ -114,6 +114,31 @@ class WgslState:
         return (start_swap + nsteps) & 1
 
 
+def wgsl_curl_rgba16f(macro4: np.ndarray) -> np.ndarray:
+    """assets/wgsl/lbm/curl_update.wgsl:12-34 in numpy: macro4 (ny, nx, 4)
+    binary32 (ux, uy, rho, w) as the collide pass wrote it; the texture holds
+    it as binary16.  curl = ((uy[x+1] - uy[max(x-1,0)]) + ux[max(y-1,0)])
+    - ux[y+1] in binary32 (left to right, as the shader writes it), with the
+    texel past the last column / row taken as zero; output rgba16float
+    (curl * 3.5 + 0.5, 0, 0, 0), float16 (ny, nx, 4)."""
+    ux = macro4[..., 0].astype(np.float16).astype(np.float32)
+    uy = macro4[..., 1].astype(np.float16).astype(np.float32)
+    ny, nx = ux.shape
+    uxp = np.zeros((ny + 1, nx), np.float32)
+    uxp[:ny] = ux
+    uyp = np.zeros((ny, nx + 1), np.float32)
+    uyp[:, :nx] = uy
+    xs, ys = np.arange(nx), np.arange(ny)
+    right = uyp[:, xs + 1]
+    left = uyp[:, np.maximum(xs - 1, 0)]
+    top = uxp[np.maximum(ys - 1, 0), :]
+    bottom = uxp[ys + 1, :]
+    curl = ((right - left) + top) - bottom
+    out = np.zeros((ny, nx, 4), np.float16)
+    out[..., 0] = (curl * np.float32(3.5) + np.float32(0.5)).astype(np.float16)
+    return out
+
+
 def wgsl_weights() -> np.ndarray:
     return np.array([lib().ws_weight(i) for i in range(9)], np.float32)
 

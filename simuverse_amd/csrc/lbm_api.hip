@@ -607,6 +607,21 @@ int lbm_read_macro_rgba16f(lbm_handle* h, uint16_t* rgba)
     return LBM_OK;
 }
 
+int lbm_read_curl_rgba16f(lbm_handle* h, uint16_t* rgba)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!rgba) return fail(LBM_ERR_INVALID, "null output");
+    uint16_t* tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, (size_t)h->n * 8));
+    hipError_t e = launch_curl_rgba16f(h->macro, h->d.nx, h->d.ny, tmp, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rgba, tmp, (size_t)h->n * 8, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(LBM_ERR_HIP, "curl: %s", hipGetErrorString(e));
+    return LBM_OK;
+}
+
 int lbm_macro(lbm_handle* h, float* rho, float* ux, float* uy)
 {
     int rc = check_handle(h);

@@ -48,6 +48,7 @@ hipError_t launch_wgsl_reduce(const WgslArgs& a, const float* src, const uint8_t
 hipError_t launch_host_to_dev(bool f16, const PhysArgs& a, const float* in, void* out, hipStream_t s);
 hipError_t launch_dev_to_host(bool f16, const PhysArgs& a, const void* in, float* out, hipStream_t s);
 hipError_t launch_pack_rgba16f(const float* macro, uint32_t n, uint16_t* out, hipStream_t s);
+hipError_t launch_curl_rgba16f(const float* macro, int nx, int ny, uint16_t* out, hipStream_t s);
 hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 struct BatchVariant {

@@ -1154,6 +1154,34 @@ __global__ __launch_bounds__(256) void k_pack_rgba16f(const float* __restrict__ 
     }
 }
 
+// assets/wgsl/lbm/curl_update.wgsl:12-34 on the macro texture: texels hold
+// ux, uy as binary16; neighbours right (min(x+1, nx), y), left
+// (max(x-1, 0), y), top (x, max(y-1, 0)), bottom (x, min(y+1, ny));
+// curl = ((uy_right - uy_left) + ux_top) - ux_bottom, stored as
+// rgba16float (curl * 3.5 + 0.5, 0, 0, 0).  The clamp to the lattice size
+// (one past the last texel) makes the last column and row read outside the
+// texture, whose value WGSL leaves to the implementation: here the zero
+// texel.
+__global__ __launch_bounds__(256) void k_curl_rgba16f(const float* __restrict__ macro, int nx, int ny,
+                                                      uint16_t* __restrict__ out)
+{
+    const uint32_t n = (uint32_t)nx * (uint32_t)ny;
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= n) return;
+    const int x = (int)(c % (uint32_t)nx), y = (int)(c / (uint32_t)nx);
+    auto texel = [&](int xx, int yy, int comp) -> float {
+        if (xx >= nx || yy >= ny) return 0.0f;
+        return (float)(_Float16)macro[(uint32_t)comp * n + (uint32_t)yy * (uint32_t)nx + (uint32_t)xx];
+    };
+    const float curl = ((texel(x + 1, y, 1) - texel(x > 0 ? x - 1 : 0, y, 1)) + texel(x, y > 0 ? y - 1 : 0, 0)) -
+                       texel(x, y + 1, 0);
+    const float r = curl * 3.5f + 0.5f;
+    out[4 * c] = __builtin_bit_cast(uint16_t, (_Float16)r);
+    out[4 * c + 1] = 0;
+    out[4 * c + 2] = 0;
+    out[4 * c + 3] = 0;
+}
+
 // Plain copy used to calibrate HBM byte counters and as the bandwidth
 // ceiling a streaming kernel reaches: one 16-byte element per thread with
 // non-temporal loads and stores (the fastest copy pattern measured on
@@ -1506,6 +1534,12 @@ hipError_t launch_dev_to_host(bool f16, const PhysArgs& a, const void* in, float
 hipError_t launch_pack_rgba16f(const float* macro, uint32_t n, uint16_t* out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_pack_rgba16f, dim3(blocks_for(n, 256)), dim3(256), 0, s, macro, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_curl_rgba16f(const float* macro, int nx, int ny, uint16_t* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_curl_rgba16f, dim3(blocks_for((size_t)nx * ny, 256)), dim3(256), 0, s, macro, nx, ny, out);
     return hipGetLastError();
 }
 

@@ -87,6 +87,7 @@ _SIGNATURES = [
     ("lbm_run_timed", C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
     ("lbm_read_macro", C.c_int, [_P, _P, _P, _P]),
     ("lbm_read_macro_rgba16f", C.c_int, [_P, _P]),
+    ("lbm_read_curl_rgba16f", C.c_int, [_P, _P]),
     ("lbm_macro", C.c_int, [_P, _P, _P, _P]),
     ("lbm_reduce", C.c_int, [_P, _P]),
     ("lbm_variant_count", C.c_int, []),
@@ -372,6 +373,13 @@ class Lattice:
     def read_macro_rgba16f(self) -> np.ndarray:
         out = np.empty((self.ny, self.nx, 4), np.uint16)
         _check(self._lib.lbm_read_macro_rgba16f(self._h, _ptr(out)))
+        return out.view(np.float16)
+
+    def read_curl_rgba16f(self) -> np.ndarray:
+        """curl_update.wgsl's texels (curl * 3.5 + 0.5, 0, 0, 0) of the
+        exported macro field, as float16 (ny, nx, 4)."""
+        out = np.empty((self.ny, self.nx, 4), np.uint16)
+        _check(self._lib.lbm_read_curl_rgba16f(self._h, _ptr(out)))
         return out.view(np.float16)
 
     def macro(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:

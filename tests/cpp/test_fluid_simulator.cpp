@@ -142,8 +142,9 @@ int main()
     o.frames(24);
     expect(same_state(node, o, true), "25 frames bit-exact");
 
-    // A click adds a disk: the band of rows goes up from the host copy.
-    sim.on_click(Vec2{300.0f, 130.0f});
+    // A click adds a disk (centre cell (200, 65)): the band of rows goes up
+    // from the host copy.
+    sim.on_click(Vec2{400.0f, 130.0f});
     {
         const uint32_t y0 = 65 - 28, y1 = y0 + 56;
         for (size_t c = (size_t)y0 * nx; c < (size_t)y1 * nx; ++c)
@@ -153,11 +154,14 @@ int main()
     o.frames(10);
     expect(same_state(node, o, true), "obstacle click, 10 frames bit-exact");
 
-    // A drag writes ExternalForce records (block_iter 90) along the path.
+    // A drag writes ExternalForce records (block_iter 90) along the path,
+    // cells (95, 62) .. (135, 66), all fluid (a record written over a solid
+    // cell would expose slots the reference's boundary pass leaves to the
+    // order of its invocations).
     sim.touch_begin();
-    sim.touch_move(Vec2{120.0f, 100.0f});
-    sim.touch_move(Vec2{150.0f, 110.0f});
-    sim.touch_move(Vec2{175.0f, 128.0f});
+    sim.touch_move(Vec2{190.0f, 124.0f});
+    sim.touch_move(Vec2{230.0f, 128.0f});
+    sim.touch_move(Vec2{270.0f, 132.0f});
     {
         std::vector<LatticeInfo> dev((size_t)nx * ny);
         if (lbm_read_lattice_info(node.handle(), dev.data(), 0, dev.size()) != LBM_OK) return 2;

@@ -159,6 +159,10 @@ def test_parity_reference_scene_with_reset_and_edits():
     np.testing.assert_array_equal(ux, st.macro[..., 0])
     rgba = lat.read_macro_rgba16f()
     np.testing.assert_array_equal(rgba, st.macro.astype(np.float16))
+    # the curl pass on those texels (curl_update.wgsl), bit for bit
+    curl = lat.read_curl_rgba16f()
+    np.testing.assert_array_equal(curl.view(np.uint16), O.wgsl_curl_rgba16f(st.macro).view(np.uint16))
+    assert np.count_nonzero(curl[..., 0] != np.float16(0.5)) > 100
 
 
 # ---------------------------------------------------------------------------

@@ -188,3 +188,57 @@ def test_shear_wave_decays_at_lattice_viscosity():
     expect = A * np.exp(-nu * k * k * steps)
     # the truncated weights also bleed momentum at ~1.6e-6 omega per step
     assert abs(amp / expect - 1) < 5e-3
+
+
+# ---------------------------------------------------------------------------
+# curl_update.wgsl restatement: closed forms on exactly representable fields
+# ---------------------------------------------------------------------------
+
+
+def test_curl_of_solid_body_rotation_is_constant_inside():
+    """u = W (-(y - c), x - c) with W = 2^-8: every interior texel sees
+    uy(x+1) - uy(x-1) = 2W and ux(y-1) - ux(y+1) = 2W, so curl = 4W and the
+    texel is 4W * 3.5 + 0.5 (all values exact in binary16/32)."""
+    ny, nx = 16, 24
+    W = 2.0**-8
+    y, x = np.mgrid[0:ny, 0:nx].astype(np.float32)
+    macro = np.zeros((ny, nx, 4), np.float32)
+    macro[..., 0] = -W * (y - 8)
+    macro[..., 1] = W * (x - 12)
+    out = O.wgsl_curl_rgba16f(macro).astype(np.float32)
+    inner = out[1:-1, 1:-1, 0]
+    assert np.all(inner == np.float32(4 * W * 3.5 + 0.5))
+    assert np.all(out[..., 1:] == 0)
+
+
+def test_curl_edges_clamp_and_zero_texel():
+    """Left / top neighbours clamp to column / row 0; right / bottom past
+    the last texel read zero: a uniform uy = a gives curl 0 inside and on
+    the first column (a - a), and -a on the last column (0 - a).""" 
+    ny, nx = 8, 10
+    a = 2.0**-6
+    macro = np.zeros((ny, nx, 4), np.float32)
+    macro[..., 1] = a
+    out = O.wgsl_curl_rgba16f(macro).astype(np.float32)[..., 0]
+    assert np.all(out[:, 1:-1] == np.float32(0.5))
+    assert np.all(out[:, 0] == np.float32(0.5))                  # uy(1) - uy(0)
+    assert np.all(out[:, -1] == np.float32(-a * 3.5 + 0.5))      # 0 - uy(nx-2)
+    # uniform ux = b: top - bottom = 0 inside, b - 0 on the last row
+    macro[..., 1] = 0
+    b = 2.0**-5
+    macro[..., 0] = b
+    out = O.wgsl_curl_rgba16f(macro).astype(np.float32)[..., 0]
+    assert np.all(out[:-1] == np.float32(0.5))
+    assert np.all(out[-1] == np.float32(b * 3.5 + 0.5))
+
+
+def test_curl_rounds_velocities_to_binary16_first():
+    """The texture stores binary16: a velocity that binary16 cannot hold
+    changes the curl exactly as its rounding says."""
+    ny, nx = 4, 5
+    macro = np.zeros((ny, nx, 4), np.float32)
+    v = np.float32(1.0 + 2.0**-12)  # rounds to 1.0 in binary16
+    macro[:, 3, 1] = v
+    out = O.wgsl_curl_rgba16f(macro).astype(np.float32)[..., 0]
+    assert out[1, 2] == np.float32(1.0 * 3.5 + 0.5)
+    assert out[1, 4] == np.float32(-1.0 * 3.5 + 0.5)
