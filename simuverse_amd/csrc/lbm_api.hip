@@ -677,7 +677,7 @@ int lbm_set_variant(lbm_handle* h, int variant)
     int rc = check_handle(h);
     if (rc) return rc;
     if (variant < 0 || variant >= kNumStepVariants) return fail(LBM_ERR_INVALID, "no variant %d", variant);
-    if (!step_variant_ok(variant, h->d.nx))
+    if (!step_variant_ok(variant, h->d.nx, h->d.x_edge))
         return fail(LBM_ERR_INVALID, "variant %d needs nx divisible by %d", variant, kStepVariants[variant].vec);
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (variant != h->variant) drop_graphs(h);
@@ -722,7 +722,7 @@ int lbm_autotune(lbm_handle* h, int steps, int* best_variant, double* ms_per_ste
     int best = h->variant;
     double best_ms = 1e300;
     for (int v = 0; v < kNumStepVariants && e == hipSuccess; ++v) {
-        if (!step_variant_ok(v, h->d.nx)) continue;
+        if (!step_variant_ok(v, h->d.nx, h->d.x_edge)) continue;
         for (int s = 0; s < 2 && e == hipSuccess; ++s) {
             e = launch_phys_step(v, h->f16, h->d.x_edge, h->force(), false, h->phys(), h->buf[h->cur],
                                  h->buf[h->cur ^ 1], h->bits, h->words, h->macro, h->stream);

@@ -64,7 +64,8 @@ __device__ inline void st(T* p, T v)
 enum : int {
     kNtStore = 1,  // non-temporal population stores
     kNtLoad = 2,   // non-temporal population loads
-    kXcdRemap = 4  // give each XCD a contiguous range of rows
+    kXcdRemap = 4, // give each XCD a contiguous range of rows
+    kShfl = 8      // aligned vector loads, x-neighbours from the adjacent lane
 };
 
 // Population storage of the physics mode.  Kernels work on deviations

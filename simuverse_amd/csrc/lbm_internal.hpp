@@ -12,11 +12,11 @@ struct StepVariant {
     int block;             // threads per block
     int cells_per_thread;  // cells handled by one thread
     int vec;               // consecutive cells per lane (1: strided by block)
-    int flags;             // kNtStore | kNtLoad | kXcdRemap
+    int flags;             // kNtStore | kNtLoad | kXcdRemap | kShfl
 };
 extern const StepVariant kStepVariants[];
 extern const int kNumStepVariants;
-bool step_variant_ok(int variant, int nx);
+bool step_variant_ok(int variant, int nx, int xedge);
 
 // Words per row of the packed solid mask (bit x+1 of a row is cell x).
 inline uint32_t solid_mask_words(int nx) { return (((uint32_t)nx + 2u) / 32u + 2u + 1u) & ~1u; }

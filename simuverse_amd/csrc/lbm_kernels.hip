@@ -450,6 +450,134 @@ __global__ __launch_bounds__(BS) void k_phys_step4(PhysArgs a, const T* __restri
     }
 }
 
+// V consecutive cells per lane with aligned V-wide loads of every direction
+// row: a wavefront covers 64*V consecutive cells of row y, and the x-shifted
+// directions take their one missing element from the adjacent lane
+// (__shfl_up / __shfl_down) or, at the wavefront's ends, from a load of the
+// cell just outside it (the same address for every lane).  Periodic x only;
+// nx must be a multiple of the block's BS*V cells.  The per-cell arithmetic
+// is phys_moments/phys_collide, so results equal phys_cell's bit for bit.
+template <typename T, bool FORCE, bool EXPORT, int V, int FLAGS>
+__device__ inline void phys_cells_shfl(const PhysArgs& a, const T* __restrict__ src, T* __restrict__ dst,
+                                       const uint32_t* __restrict__ bits, uint32_t words,
+                                       float* __restrict__ macro, uint32_t n, int wbase, int lane, int y)
+{
+    typedef T vecT __attribute__((ext_vector_type(V)));
+    const int nx = a.nx, ny = a.ny;
+    const int x0 = wbase + lane * V;
+    const int ym = (y == 0) ? ny - 1 : y - 1;
+    const int yp = (y == ny - 1) ? 0 : y + 1;
+    const uint32_t r0 = (uint32_t)y * a.rs, rm = (uint32_t)ym * a.rs, rp = (uint32_t)yp * a.rs;
+    const int xl = wbase == 0 ? nx - 1 : wbase - 1;
+    const int xr = wbase + 64 * V == nx ? 0 : wbase + 64 * V;
+    float g[9][V];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int ex = ex_of(i), ey = ey_of(i);
+        const T* row = src + (ey == 1 ? rm : (ey == -1 ? rp : r0)) + (uint32_t)i * a.plane;
+        vecT v;
+        if constexpr ((FLAGS & kNtLoad) != 0) v = __builtin_nontemporal_load(reinterpret_cast<const vecT*>(row + x0));
+        else v = *reinterpret_cast<const vecT*>(row + x0);
+        float f[V];
+#pragma unroll
+        for (int k = 0; k < V; ++k) f[k] = Store<T>::from_storage(v[k], a.inv_scale);
+        if (ex == 0) {
+#pragma unroll
+            for (int k = 0; k < V; ++k) g[i][k] = f[k];
+        } else if (ex == 1) {  // pulls from x - 1
+            float prev = __shfl_up(f[V - 1], 1, 64);
+            const float edge = Store<T>::from_storage(row[xl], a.inv_scale);
+            g[i][0] = lane == 0 ? edge : prev;
+#pragma unroll
+            for (int k = 1; k < V; ++k) g[i][k] = f[k - 1];
+        } else {  // pulls from x + 1
+            float next = __shfl_down(f[0], 1, 64);
+            const float edge = Store<T>::from_storage(row[xr], a.inv_scale);
+#pragma unroll
+            for (int k = 0; k < V - 1; ++k) g[i][k] = f[k + 1];
+            g[i][V - 1] = lane == 63 ? edge : next;
+        }
+    }
+    // Solid bits of cells x0-1 .. x0+V of rows ym, y, yp (bit x+1 of a mask
+    // row is cell x; x0 % 32 + V + 2 <= 64 keeps the window in two words).
+    auto window = [&](int yy) {
+        const uint32_t* p = bits + (uint32_t)yy * words + ((uint32_t)x0 >> 5);
+        const uint64_t w64 = (uint64_t)p[0] | ((uint64_t)p[1] << 32);
+        return (uint32_t)(w64 >> ((uint32_t)x0 & 31u)) & ((1u << (V + 2)) - 1u);
+    };
+    const uint32_t sm = window(ym), s0 = window(y), sp = window(yp);
+    const uint32_t self = (s0 >> 1) & ((1u << V) - 1u);  // bit k: cell x0 + k is solid
+    const uint32_t c0 = (uint32_t)y * (uint32_t)nx + (uint32_t)x0;
+    const uint32_t p0 = r0 + (uint32_t)x0;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        float gc[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) gc[i] = g[i][k];
+        float rho, ux, uy;
+        phys_moments<FORCE>(a, gc, rho, ux, uy);
+        if (EXPORT) {
+            const bool sk = (self >> k) & 1u;
+            macro[c0 + k] = sk ? 0.0f : ux;
+            macro[c0 + k + n] = sk ? 0.0f : uy;
+            macro[c0 + k + 2 * n] = sk ? 0.0f : rho;
+            macro[c0 + k + 3 * n] = sk ? 0.0f : 1.0f;
+        }
+        phys_collide<FORCE>(a, gc, rho, ux, uy);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) g[i][k] = gc[i];
+    }
+    if (self == 0u) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            vecT o;
+#pragma unroll
+            for (int k = 0; k < V; ++k) o[k] = Store<T>::to_storage(g[i][k], a.scale);
+            vecT* q = reinterpret_cast<vecT*>(dst + p0 + (uint32_t)i * a.plane);
+            if constexpr ((FLAGS & kNtStore) != 0) __builtin_nontemporal_store(o, q);
+            else *q = o;
+        }
+    } else {
+        // A solid cell's slots receive pushes from its neighbours during
+        // this step: write the fluid cells only.
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            if ((self >> k) & 1u) continue;
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                Store<T>::store_dev(dst, p0 + (uint32_t)k + (uint32_t)i * a.plane, g[i][k], w_phys(i), a.scale);
+        }
+    }
+    if ((sm | sp | s0) == 0u) return;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        if ((self >> k) & 1u) continue;
+#pragma unroll
+        for (int i = 1; i < 9; ++i) {
+            const int ex = ex_of(i), ey = ey_of(i);
+            const uint32_t srow = ey == 1 ? sp : (ey == -1 ? sm : s0);
+            if (!((srow >> (k + 1 + ex)) & 1u)) continue;
+            int tx = x0 + k + ex;
+            tx = tx < 0 ? nx - 1 : (tx >= nx ? 0 : tx);
+            const uint32_t trow = ey == 1 ? rp : (ey == -1 ? rm : r0);
+            Store<T>::store_dev(dst, trow + (uint32_t)tx + (uint32_t)inv_of(i) * a.plane, g[i][k], w_phys(i),
+                                a.scale);
+        }
+    }
+}
+
+template <typename T, bool FORCE, bool EXPORT, int BS, int V, int FLAGS>
+__global__ __launch_bounds__(BS) void k_phys_step_shfl(PhysArgs a, const T* __restrict__ src, T* __restrict__ dst,
+                                                       const uint32_t* __restrict__ bits, uint32_t words,
+                                                       float* __restrict__ macro, uint32_t nseg)
+{
+    const uint32_t tile = blockIdx.x;
+    const int y = (int)(tile / nseg);
+    const int wbase = (int)(tile % nseg) * (BS * V) + (int)(threadIdx.x >> 6) * 64 * V;
+    phys_cells_shfl<T, FORCE, EXPORT, V, FLAGS>(a, src, dst, bits, words, macro, (uint32_t)a.nx * (uint32_t)a.ny,
+                                                wbase, (int)(threadIdx.x & 63u), y);
+}
+
 // ---------------------------------------------------------------------------
 // Batched domains: one persistent launch steps every domain.
 // ---------------------------------------------------------------------------
@@ -1222,6 +1350,23 @@ static hipError_t launch_phys_variant4(const PhysArgs& a, const void* src, void*
     return hipGetLastError();
 }
 
+template <typename T, int XEDGE, bool FORCE, bool EXPORT, int BS, int V, int FLAGS>
+static hipError_t launch_phys_variant_shfl(const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
+                                           uint32_t words, float* macro, hipStream_t s)
+{
+    if constexpr (XEDGE != 0) {
+        return hipErrorInvalidValue;
+    } else {
+        // whole blocks of BS*V cells per row: every lane's V-wide access and
+        // both edge loads stay inside the row
+        if (a.nx % (BS * V) != 0) return hipErrorInvalidValue;
+        const uint32_t nseg = (uint32_t)(a.nx / (BS * V));
+        hipLaunchKernelGGL((k_phys_step_shfl<T, FORCE, EXPORT, BS, V, FLAGS>), dim3(nseg * (uint32_t)a.ny), dim3(BS),
+                           0, s, a, (const T*)src, (T*)dst, bits, words, macro, nseg);
+        return hipGetLastError();
+    }
+}
+
 // Tile shapes the step kernel is built for: threads per block, cells per
 // thread, consecutive cells per lane (4 needs nx % 4 == 0) and the kNt* /
 // kXcdRemap flags.  Every shape computes the same bits.
@@ -1238,22 +1383,32 @@ const StepVariant kStepVariants[] = {
     {64, 4, 4, kNtStore},
     {128, 4, 4, 0},
     {1024, 1, 1, kNtStore},
+    {256, 1, 4, kNtStore | kShfl},
+    {128, 1, 4, kNtStore | kShfl},
+    {256, 1, 2, kNtStore | kShfl},
+    {256, 1, 8, kNtStore | kShfl},
+    {64, 1, 8, kNtStore | kShfl},
+    {512, 1, 4, kNtStore | kShfl},
 };
 const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
 
-bool step_variant_ok(int v, int nx)
+bool step_variant_ok(int v, int nx, int xedge)
 {
     if (v < 0 || v >= kNumStepVariants) return false;
-    return kStepVariants[v].vec == 1 || nx % kStepVariants[v].vec == 0;
+    const StepVariant& sv = kStepVariants[v];
+    if (sv.flags & kShfl) return xedge == 0 && nx % (sv.block * sv.vec) == 0;
+    return sv.vec == 1 || nx % sv.vec == 0;
 }
 
 template <typename T, int XEDGE, bool FORCE, bool EXPORT>
 static hipError_t dispatch_variant(int v, const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
                                    uint32_t words, float* macro, hipStream_t s)
 {
-    constexpr int S = kNtStore, L = kNtLoad, R = kXcdRemap;
+    constexpr int S = kNtStore, L = kNtLoad, R = kXcdRemap, H = kShfl;
 #define LBM_V(BS, CPT, FL) return launch_phys_variant<T, XEDGE, FORCE, EXPORT, BS, CPT, FL>(a, src, dst, bits, words, macro, s)
 #define LBM_V4(BS, FL) return launch_phys_variant4<T, XEDGE, FORCE, EXPORT, BS, FL>(a, src, dst, bits, words, macro, s)
+#define LBM_VS(BS, V, FL) \
+    return launch_phys_variant_shfl<T, XEDGE, FORCE, EXPORT, BS, V, FL>(a, src, dst, bits, words, macro, s)
     switch (v) {
     case 0: LBM_V(256, 1, 0);
     case 1: LBM_V(256, 1, S);
@@ -1267,10 +1422,17 @@ static hipError_t dispatch_variant(int v, const PhysArgs& a, const void* src, vo
     case 9: LBM_V4(64, S);
     case 10: LBM_V4(128, 0);
     case 11: LBM_V(1024, 1, S);
+    case 12: LBM_VS(256, 4, S | H);
+    case 13: LBM_VS(128, 4, S | H);
+    case 14: LBM_VS(256, 2, S | H);
+    case 15: LBM_VS(256, 8, S | H);
+    case 16: LBM_VS(64, 8, S | H);
+    case 17: LBM_VS(512, 4, S | H);
     default: return hipErrorInvalidValue;
     }
 #undef LBM_V
 #undef LBM_V4
+#undef LBM_VS
 }
 
 template <typename T, int XEDGE>

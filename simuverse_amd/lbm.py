@@ -193,7 +193,8 @@ def device_count() -> int:
 
 def variant_shapes() -> list[tuple[int, int, int, int]]:
     """(threads per block, cells per thread, consecutive cells per lane,
-    flags: 1 non-temporal stores, 2 non-temporal loads, 4 rows per XCD)."""
+    flags: 1 non-temporal stores, 2 non-temporal loads, 4 rows per XCD,
+    8 aligned vector loads with x-neighbours from the adjacent lane)."""
     lib = load_library()
     out = []
     for v in range(lib.lbm_variant_count()):
@@ -201,6 +202,16 @@ def variant_shapes() -> list[tuple[int, int, int, int]]:
         _check(lib.lbm_variant_info(v, C.byref(b), C.byref(c), C.byref(w), C.byref(f)))
         out.append((b.value, c.value, w.value, f.value))
     return out
+
+
+def variant_ok(variant: int, nx: int, x_edge: int = X_PERIODIC) -> bool:
+    """Whether a tile variant can step an nx-wide lattice (the library's
+    own rule: lane-shuffle variants need periodic x and whole blocks of
+    block*vec cells per row; vector variants need nx % vec == 0)."""
+    b, _, v, f = variant_shapes()[variant]
+    if f & 8:
+        return x_edge == X_PERIODIC and nx % (b * v) == 0
+    return v == 1 or nx % v == 0
 
 
 def copy_bandwidth(nbytes: int, reps: int = 20) -> float:

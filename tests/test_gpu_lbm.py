@@ -262,29 +262,35 @@ def test_physics_reduce_identities_small():
     assert abs(lat.reduce()[0] - m0) < 1e-6 * m0
 
 
-def test_tile_variants_and_graphs_bit_identical():
-    """Every tile variant, with and without graphs, in both device layouts
-    (row pitch 304 > nx = 300 in the row-interleaved one) gives the same
-    populations bit for bit."""
-    print("case: variants + graphs 300x200", flush=True)
-    nx, ny = 300, 200
+@pytest.mark.parametrize("storage", [L.STORAGE_F32, L.STORAGE_F16_DEV], ids=["f32", "f16"])
+@pytest.mark.parametrize("nx, ny", [(300, 200), (2048, 40)])
+def test_tile_variants_and_graphs_bit_identical(storage, nx, ny):
+    """Every tile variant that can step the lattice, with and without
+    graphs, in both device layouts (row pitch 304 > nx = 300 in the
+    row-interleaved one) gives the same populations bit for bit; the
+    2048-wide lattice admits the lane-shuffle variants."""
+    print(f"case: variants + graphs {nx}x{ny} storage {storage}", flush=True)
     scene = W.periodic_disks(nx, ny, 0.05, 4, 10, seed=4)
     f0 = W.rest_state(nx, ny, scene.solid, amplitude=1e-3).astype(np.float32)
-    outs = []
+    outs, names = [], []
     nv = len(L.variant_shapes())
-    for v in range(2 * nv):
-        layout = L.LAYOUT_ROWS if v < nv else L.LAYOUT_PLANES
-        v %= nv
-        lat = L.Lattice(nx, ny, 0.58, force=(1e-5, 0.0), layout=layout)
-        lat.write_materials(W.materials_from_solid(scene.solid))
-        lat.write_populations(f0, 0)
-        lat.set_variant(v)
-        lat.set_graph_chunk(0 if v % 2 else 100)
-        lat.run(202)
-        outs.append(lat.read_populations())
-        lat.close()
+    for layout in (L.LAYOUT_ROWS, L.LAYOUT_PLANES):
+        for v in range(nv):
+            if not L.variant_ok(v, nx):
+                continue
+            lat = L.Lattice(nx, ny, 0.58, force=(1e-5, 0.0), layout=layout, storage=storage)
+            lat.write_materials(W.materials_from_solid(scene.solid))
+            lat.write_populations(f0, 0)
+            lat.set_variant(v)
+            lat.set_graph_chunk(0 if v % 2 else 100)
+            lat.run(202)
+            outs.append(lat.read_populations())
+            names.append((layout, v, L.variant_shapes()[v]))
+            lat.close()
+    if nx % 1024 == 0:
+        assert any(n[2][3] & 8 for n in names), "no lane-shuffle variant ran"
     for k, o in enumerate(outs[1:], 1):
-        assert np.array_equal(o, outs[0]), f"variant {k} differs"
+        assert np.array_equal(o, outs[0]), f"variant {names[k]} differs from {names[0]}"
 
 
 def test_checkpoint_resume_bit_identical():
