@@ -86,6 +86,19 @@ def cpu_baseline(scene, budget_s: float = 12.0):
     }
 
 
+def describe_variant(shape):
+    """The step-kernel variant as a dict: one-step shapes (block,
+    cells_per_thread, vec) or the two-step LDS tile (rows x width)."""
+    b, c, v, f = shape
+    flags = [name for bit, name in ((1, "nt_store"), (2, "nt_load"), (4, "xcd_rows"), (8, "lane_shuffle"),
+                                    (16, "two_step")) if f & bit]
+    if f & 16:
+        return {"kernel": "k_phys_step2", "block": b, "tile_rows": c, "tile_width": v, "flags": flags,
+                "steps_per_launch": 2}
+    kern = "k_phys_step_shfl" if f & 8 else ("k_phys_step4" if v == 4 else "k_phys_step")
+    return {"kernel": kern, "block": b, "cells_per_thread": c, "vec": v, "flags": flags, "steps_per_launch": 1}
+
+
 def run_copy_calibration(nbytes: int):
     from simuverse_amd import lbm as L
 
@@ -213,6 +226,7 @@ def run_learned(args):
         net.add_transition(a, b, solid_c, tau_c, gc, stride=stride, offset=pairs % stride)
         pairs += 1
     fine2.close()
+    rx, rt = net.normalize_samples()
     t_collect = time.perf_counter() - t0
     t0 = time.perf_counter()
     loss = net.train(steps=args.learned_train_steps, batch=min(args.learned_batch, net.sample_count()), lr=2e-3,
@@ -416,7 +430,7 @@ def main():
             "disks": int(len(scene.disks)),
             "seed": W.SEED,
             "layout": args.layout,
-            "tile_variant": dict(zip(("block", "cells_per_thread", "vec", "flags"), L.variant_shapes()[variant])),
+            "tile_variant": describe_variant(L.variant_shapes()[variant]),
             "graph_chunk": args.graph_chunk,
             "wall_ms": round(wall * 1e3, 3),
         },

@@ -297,9 +297,12 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* best_variant, double* ms_pe
  * No reference counterpart.  A per-cell network 9 -> 64 -> 64 -> 9 (GELU,
  * erf form) maps the non-equilibrium part of the gathered populations,
  * x_i = (f_i - feq_i) / (w_i * in_scale), to a correction
- * df_i = w_i * out_scale * y_i; the correction is projected onto the
- * non-conserved moments (its mass and momentum are removed) and added to the
- * BGK post-collision populations.  Parameters: one vector of 5385 floats,
+ * df_i = w_i * out_scale * clamp(y_i, +-8); the correction is projected
+ * onto the second-order non-conserved moments (energy e and deviatoric
+ * stresses p_xx, p_xy of the orthogonal moment basis: no mass, no momentum,
+ * no ghost moments) and added to the BGK post-collision populations of
+ * cells whose 3x3 neighbourhood is fluid (the cells the network is trained
+ * on).  Parameters: one vector of 5385 floats,
  * W1[64][9] b1[64] W2[64][64] b2[64] W3[9][64] b3[9] (row = output unit).
  * Training (forward, backward, Adam) runs on the device in binary32. */
 typedef struct lbm_mlp lbm_mlp;
@@ -327,10 +330,15 @@ int lbm_mlp_add_samples(lbm_mlp* m, const float* x, const float* t, int n);
 /* Samples from two consecutive pre-collision states of a periodic lattice
  * (dense f[i*nx*ny + c], binary32): for every stride-th cell from offset
  * whose 3x3 neighbourhood is fluid, x = the network input at time t and
- * t = the projected residual f_i(c + e_i, t+1) - BGK_i(f(c, t)) in output
- * units.  Uses the current scales. */
+ * t = the residual f_i(c + e_i, t+1) - BGK_i(f(c, t)) projected like the
+ * correction, in output units.  Uses the current scales. */
 int lbm_mlp_add_transition(lbm_mlp* m, const float* state_t, const float* state_t1, const uint8_t* solid, int nx,
                            int ny, float tau, float force_x, float force_y, int stride, int offset, int* added);
+/* Rescales the stored samples to unit root-mean-square (one factor for all
+ * inputs, one for all targets) and folds the factors into the network's
+ * scales, so a network trained afterwards sees order-one data; returns the
+ * factors.  Call after adding samples and before training. */
+int lbm_mlp_normalize_samples(lbm_mlp* m, float* in_rms, float* out_rms);
 /* Adam (beta1 0.9, beta2 0.999, eps 1e-8) over `steps` minibatches of
  * `batch` samples drawn from the stored ones with SplitMix64(seed). */
 int lbm_mlp_train(lbm_mlp* m, int steps, int batch, float lr, uint64_t seed, double* last_loss);

@@ -126,6 +126,8 @@ namespace {
 
 void drop_graphs(lbm_handle* h)
 {
+    // a graph launched on the stream may still run: let it finish first
+    if (h->graph[0] || h->graph[1]) (void)hipStreamSynchronize(h->stream);
     for (auto& g : h->graph) {
         if (g) (void)hipGraphExecDestroy(g);
         g = nullptr;
@@ -155,20 +157,41 @@ hipError_t enqueue_step(lbm_handle* h, int swap, bool export_macro)
         return launch_phys_step_learned(h->f16, h->d.x_edge, h->force(), h->phys(), src, dst, h->bits, h->words,
                                         lbm_mlp_device_params(h->mlp), s_in, s_out, h->stream);
     }
-    if (h->physics())
-        return launch_phys_step(h->variant, h->f16, h->d.x_edge, h->force(), export_macro, h->phys(), src, dst,
-                                h->bits, h->words, h->macro, h->stream);
+    if (h->physics()) {
+        const int v = step_variant_two_step(h->variant) ? kSingleStepFallback : h->variant;
+        return launch_phys_step(v, h->f16, h->d.x_edge, h->force(), export_macro, h->phys(), src, dst, h->bits,
+                                h->words, h->macro, h->stream);
+    }
     return launch_wgsl_step(export_macro, h->wgsl(), (const float*)src, (float*)dst, h->mat, h->biter, h->fvel,
                             h->macro, h->stream);
 }
+
+// Whether steps go two per launch (a two-step variant on a plain physics
+// handle).
+bool two_step(const lbm_handle* h) { return h->physics() && !h->mlp && step_variant_two_step(h->variant); }
+
+// Two steps from buffer `swap` into buffer 1-swap in one launch.
+hipError_t enqueue_step2(lbm_handle* h, int swap)
+{
+    return launch_phys_step2(h->variant, h->f16, h->d.x_edge, h->force(), h->phys(), h->buf[swap],
+                             h->buf[swap ^ 1], h->bits, h->words, h->stream);
+}
+
+// Buffer flips of one graph chunk: one per launch.
+int chunk_flips(const lbm_handle* h) { return two_step(h) ? (h->graph_chunk / 2) & 1 : 0; }
 
 int ensure_graph(lbm_handle* h, int b)
 {
     if (h->graph[b]) return LBM_OK;
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     hipError_t launch_err = hipSuccess;
-    for (int s = 0; s < h->graph_chunk && launch_err == hipSuccess; ++s)
-        launch_err = enqueue_step(h, (b + s) & 1, false);
+    if (two_step(h)) {
+        for (int s = 0; s < h->graph_chunk / 2 && launch_err == hipSuccess; ++s)
+            launch_err = enqueue_step2(h, (b + s) & 1);
+    } else {
+        for (int s = 0; s < h->graph_chunk && launch_err == hipSuccess; ++s)
+            launch_err = enqueue_step(h, (b + s) & 1, false);
+    }
     hipGraph_t g = nullptr;
     hipError_t end_err = hipStreamEndCapture(h->stream, &g);
     if (launch_err != hipSuccess || end_err != hipSuccess) {
@@ -194,7 +217,14 @@ int enqueue_run(lbm_handle* h, int nsteps, bool export_macro)
             int rc = ensure_graph(h, h->cur);
             if (rc != LBM_OK) return rc;
             HIP_TRY(hipGraphLaunch(h->graph[h->cur], h->stream));
-            plain -= h->graph_chunk;  // chunk is even: the current buffer is unchanged
+            plain -= h->graph_chunk;
+            h->cur ^= chunk_flips(h);  // one flip per launch: even for one-step chunks
+        }
+    }
+    if (two_step(h)) {
+        for (; plain >= 2; plain -= 2) {
+            HIP_TRY(enqueue_step2(h, h->cur));
+            h->cur ^= 1;
         }
     }
     for (int s = 0; s < plain; ++s) {
@@ -677,7 +707,7 @@ int lbm_set_variant(lbm_handle* h, int variant)
     int rc = check_handle(h);
     if (rc) return rc;
     if (variant < 0 || variant >= kNumStepVariants) return fail(LBM_ERR_INVALID, "no variant %d", variant);
-    if (!step_variant_ok(variant, h->d.nx, h->d.x_edge))
+    if (!step_variant_ok(variant, h->d.nx, h->d.ny, h->d.x_edge))
         return fail(LBM_ERR_INVALID, "variant %d needs nx divisible by %d", variant, kStepVariants[variant].vec);
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (variant != h->variant) drop_graphs(h);
@@ -722,24 +752,28 @@ int lbm_autotune(lbm_handle* h, int steps, int* best_variant, double* ms_per_ste
     int best = h->variant;
     double best_ms = 1e300;
     for (int v = 0; v < kNumStepVariants && e == hipSuccess; ++v) {
-        if (!step_variant_ok(v, h->d.nx, h->d.x_edge)) continue;
-        for (int s = 0; s < 2 && e == hipSuccess; ++s) {
-            e = launch_phys_step(v, h->f16, h->d.x_edge, h->force(), false, h->phys(), h->buf[h->cur],
-                                 h->buf[h->cur ^ 1], h->bits, h->words, h->macro, h->stream);
+        if (!step_variant_ok(v, h->d.nx, h->d.ny, h->d.x_edge)) continue;
+        const bool ts = step_variant_two_step(v);
+        auto launch = [&]() {
+            hipError_t r = ts ? launch_phys_step2(v, h->f16, h->d.x_edge, h->force(), h->phys(), h->buf[h->cur],
+                                                  h->buf[h->cur ^ 1], h->bits, h->words, h->stream)
+                              : launch_phys_step(v, h->f16, h->d.x_edge, h->force(), false, h->phys(),
+                                                 h->buf[h->cur], h->buf[h->cur ^ 1], h->bits, h->words, h->macro,
+                                                 h->stream);
             h->cur ^= 1;
-        }
+            return r;
+        };
+        const int per = ts ? 2 : 1;  // steps per launch
+        for (int s = 0; s < 2 && e == hipSuccess; ++s) e = launch();
         if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
-        for (int s = 0; s < steps && e == hipSuccess; ++s) {
-            e = launch_phys_step(v, h->f16, h->d.x_edge, h->force(), false, h->phys(), h->buf[h->cur],
-                                 h->buf[h->cur ^ 1], h->bits, h->words, h->macro, h->stream);
-            h->cur ^= 1;
-        }
+        const int launches = (steps + per - 1) / per;
+        for (int s = 0; s < launches && e == hipSuccess; ++s) e = launch();
         if (e == hipSuccess) e = hipEventRecord(h->ev1, h->stream);
         if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
         float t = 0.0f;
         if (e == hipSuccess) e = hipEventElapsedTime(&t, h->ev0, h->ev1);
-        if (e == hipSuccess && t / steps < best_ms) {
-            best_ms = t / steps;
+        if (e == hipSuccess && t / (launches * per) < best_ms) {
+            best_ms = t / (launches * per);
             best = v;
         }
     }

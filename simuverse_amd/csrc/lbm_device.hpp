@@ -65,7 +65,8 @@ enum : int {
     kNtStore = 1,  // non-temporal population stores
     kNtLoad = 2,   // non-temporal population loads
     kXcdRemap = 4, // give each XCD a contiguous range of rows
-    kShfl = 8      // aligned vector loads, x-neighbours from the adjacent lane
+    kShfl = 8,     // aligned vector loads, x-neighbours from the adjacent lane
+    kTwoStep = 16  // two steps per launch through an LDS tile (temporal blocking)
 };
 
 // Population storage of the physics mode.  Kernels work on deviations
@@ -174,6 +175,10 @@ constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH;
 constexpr int kOffW3 = kOffB2 + kMlpH;
 constexpr int kOffB3 = kOffW3 + kMlpOut * kMlpH;
 constexpr int kMlpParams = kOffB3 + kMlpOut;  // 5385
+// The step clamps network outputs to +-kMlpOutClamp: training targets are
+// normalised to unit root-mean-square, so this bounds a correction to a few
+// times the largest the network was trained to produce.
+constexpr float kMlpOutClamp = 8.0f;
 
 __device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
 __device__ inline float gelu_grad(float x)
@@ -360,6 +365,37 @@ __device__ inline void project_correction(float d[9])
     }
 #pragma unroll
     for (int i = 0; i < 9; ++i) d[i] -= w_phys(i) * (a + 3.0f * ((float)ex_of(i) * bx + (float)ey_of(i) * by));
+}
+
+// Projection of a collision correction onto the second-order non-conserved
+// moments: the energy e and the deviatoric stresses p_xx, p_xy of the
+// orthogonal D2Q9 moment basis (Lallemand & Luo, Phys. Rev. E 61 (2000)
+// 6546), rows
+//   e_i    = -4 (rest), -1 (axis), 2 (diagonal)       |e|^2    = 36
+//   pxx_i  = ex^2 - ey^2                              |pxx|^2  = 4
+//   pxy_i  = ex ey                                    |pxy|^2  = 4
+// The rows are orthogonal to each other and to 1, ex, ey, so the result
+// carries no mass and no momentum, and the higher (ghost) moments that do
+// not enter the momentum flux are left to BGK.
+__host__ __device__ constexpr float mom_e(int i) { return i == 0 ? -4.0f : (i <= 4 ? -1.0f : 2.0f); }
+__device__ inline void project_stress(float d[9])
+{
+    float me = 0.0f, mxx = 0.0f, mxy = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float ex = (float)ex_of(i), ey = (float)ey_of(i);
+        me += mom_e(i) * d[i];
+        mxx += (ex * ex - ey * ey) * d[i];
+        mxy += (ex * ey) * d[i];
+    }
+    me *= 1.0f / 36.0f;
+    mxx *= 0.25f;
+    mxy *= 0.25f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float ex = (float)ex_of(i), ey = (float)ey_of(i);
+        d[i] = mom_e(i) * me + (ex * ex - ey * ey) * mxx + (ex * ey) * mxy;
+    }
 }
 
 // Linear block id -> logical tile id such that the blocks the hardware

@@ -12,11 +12,16 @@ struct StepVariant {
     int block;             // threads per block
     int cells_per_thread;  // cells handled by one thread
     int vec;               // consecutive cells per lane (1: strided by block)
-    int flags;             // kNtStore | kNtLoad | kXcdRemap | kShfl
+    int flags;             // kNtStore | kNtLoad | kXcdRemap | kShfl | kTwoStep
 };
+// kTwoStep variants: cells_per_thread = tile rows, vec = tile width; one
+// launch advances two steps (launch_phys_step2), and single steps use the
+// one-step variant kSingleStepFallback.
+constexpr int kSingleStepFallback = 6;
 extern const StepVariant kStepVariants[];
 extern const int kNumStepVariants;
-bool step_variant_ok(int variant, int nx, int xedge);
+bool step_variant_ok(int variant, int nx, int ny, int xedge);
+inline bool step_variant_two_step(int variant) { return (kStepVariants[variant].flags & kTwoStep) != 0; }
 
 // Words per row of the packed solid mask (bit x+1 of a row is cell x).
 inline uint32_t solid_mask_words(int nx) { return (((uint32_t)nx + 2u) / 32u + 2u + 1u) & ~1u; }
@@ -25,6 +30,8 @@ hipError_t launch_build_solid_bits(const uint8_t* mat, int nx, int ny, int perio
 hipError_t launch_phys_step(int variant, bool f16, int xedge, bool force, bool export_macro, const PhysArgs& a,
                             const void* src, void* dst, const uint32_t* bits, uint32_t words, float* macro,
                             hipStream_t s);
+hipError_t launch_phys_step2(int variant, bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
+                             void* dst, const uint32_t* bits, uint32_t words, hipStream_t s);
 hipError_t launch_phys_macro(bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
                              const uint8_t* mat, float* rho, float* ux, float* uy, hipStream_t s);
 hipError_t launch_phys_prime(bool f16, int xedge, const PhysArgs& a, void* buf, const uint8_t* mat,

@@ -258,16 +258,22 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
         macro[c + 3 * n] = 1.0f;
     }
     float corr[9];
+    bool learned = false;
     if constexpr (LEARNED) {
-        float xin[9];
-        cell_features(g, rho, ux, uy, s_in, xin);
-        mlp_forward_one(mlp, xin, corr);
+        // The network is trained on cells whose 3x3 neighbourhood is fluid
+        // (away from walls and open edges); only those get the correction.
+        learned = (nb.sm | nb.s0 | nb.sp) == 0u && (XEDGE == 0 || (x > 0 && x < a.nx - 1));
+        if (learned) {
+            float xin[9];
+            cell_features(g, rho, ux, uy, s_in, xin);
+            mlp_forward_one(mlp, xin, corr);
 #pragma unroll
-        for (int i = 0; i < 9; ++i) corr[i] *= w_phys(i) * s_out;
-        project_correction(corr);
+            for (int i = 0; i < 9; ++i) corr[i] = fminf(fmaxf(corr[i], -kMlpOutClamp), kMlpOutClamp) * (w_phys(i) * s_out);
+            project_stress(corr);
+        }
     }
     phys_collide<FORCE>(a, g, rho, ux, uy);
-    if constexpr (LEARNED) {
+    if (learned) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) g[i] += corr[i];
     }
@@ -578,6 +584,76 @@ __global__ __launch_bounds__(BS) void k_phys_step_shfl(PhysArgs a, const T* __re
                                                 wbase, (int)(threadIdx.x & 63u), y);
 }
 
+// Two steps per launch (temporal blocking), periodic x.  A block owns a
+// BW x BH tile of cells.  Step 1 runs on the tile plus a one-cell ring,
+// gathering from HBM exactly as the one-step kernel does, and leaves each
+// cell's post-collision populations in LDS (rounded to the storage type, as
+// the one-step path rounds them into its buffer).  Step 2 runs on the tile,
+// gathering from LDS: a population that would come from a solid cell is the
+// cell's own post-collision population of the opposite direction (the
+// half-way bounce-back the push rule produces), so step 1 needs no pushes.
+// Step 2 stores to HBM with the usual pushes into solid neighbours, so the
+// destination buffer ends up exactly as two one-step launches leave it.
+// HBM traffic per two steps: the ring's extra reads ((BW+2)(BH+2)/(BW BH)
+// of 36 B) plus 36 B written, instead of 2 x 72 B.
+template <typename T, bool FORCE, int BW, int BH, int FLAGS>
+__global__ __launch_bounds__(256) void k_phys_step2(PhysArgs a, const T* __restrict__ src, T* __restrict__ dst,
+                                                    const uint32_t* __restrict__ bits, uint32_t words,
+                                                    uint32_t tiles_x)
+{
+    constexpr int RW = BW + 2, RH = BH + 2, RC = RW * RH;  // step-1 region
+    constexpr int kIter1 = (RC + 255) / 256, kIter2 = (BW * BH) / 256;
+    static_assert((BW * BH) % 256 == 0, "tile must be a multiple of the block");
+    __shared__ T lds[9][RC];
+    const int x0 = (int)(blockIdx.x % tiles_x) * BW;
+    const int y0 = (int)(blockIdx.x / tiles_x) * BH;
+    const int nx = a.nx, ny = a.ny;
+    // Step 1: region cells (q, r) = (x0 - 1 + q, y0 - 1 + r), wrapped.
+#pragma unroll
+    for (int m = 0; m < kIter1; ++m) {
+        const int k = (int)threadIdx.x + 256 * m;
+        if (k >= RC) break;
+        const int r = k / RW, q = k % RW;
+        int x = x0 - 1 + q, y = y0 - 1 + r;
+        x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+        const CellNbr nb = cell_nbr(a, bits, words, x, y);
+        if (nb.s0 & 2u) continue;  // solid: never read by step 2
+        float g[9];
+        cell_gather<T, 0, FLAGS>(a, src, nb, x, y, g);
+        float rho, ux, uy;
+        phys_moments<FORCE>(a, g, rho, ux, uy);
+        phys_collide<FORCE>(a, g, rho, ux, uy);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) lds[i][k] = Store<T>::to_storage(g[i], a.scale);
+    }
+    __syncthreads();
+    // Step 2: tile cells, gathered from LDS.
+#pragma unroll
+    for (int m = 0; m < kIter2; ++m) {
+        const int k = (int)threadIdx.x + 256 * m;
+        const int row = k / BW, col = k % BW;
+        const int x = x0 + col, y = y0 + row;
+        const CellNbr nb = cell_nbr(a, bits, words, x, y);
+        if (nb.s0 & 2u) continue;
+        const int c = (row + 1) * RW + (col + 1);
+        float g[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const int ex = ex_of(i), ey = ey_of(i);
+            // source cell x - e_i: bit (1 - ex) of row y - ey
+            const uint32_t srow = ey == 1 ? nb.sm : (ey == -1 ? nb.sp : nb.s0);
+            const bool from_solid = (srow >> (1 - ex)) & 1u;
+            const T v = from_solid ? lds[inv_of(i)][c] : lds[i][c - ey * RW - ex];
+            g[i] = Store<T>::from_storage(v, a.inv_scale);
+        }
+        float rho, ux, uy;
+        phys_moments<FORCE>(a, g, rho, ux, uy);
+        phys_collide<FORCE>(a, g, rho, ux, uy);
+        cell_store<T, 0, FLAGS>(a, dst, nb, x, y, g);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Batched domains: one persistent launch steps every domain.
 // ---------------------------------------------------------------------------
@@ -780,19 +856,22 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
     for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
         const int y = (int)(tile / nseg);
         const int x = (int)(tile % nseg) * 256 + (int)threadIdx.x;
-        bool fluid = false;
+        bool fluid = false, learned = false;
         CellNbr nb{};
         float g[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         float rho = 1.0f, ux = 0.0f, uy = 0.0f;
         if (x < a.nx) {
             nb = cell_nbr(a, bits, words, x, y);
             fluid = !(nb.s0 & 2u);
+            // trained on cells whose 3x3 neighbourhood is fluid: only those
+            // get the correction
+            learned = (nb.sm | nb.s0 | nb.sp) == 0u && (XEDGE == 0 || (x > 0 && x < a.nx - 1));
         }
         float xin[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         if (fluid) {
             cell_gather<T, XEDGE, kNtStore>(a, src, nb, x, y, g);
             phys_moments<FORCE>(a, g, rho, ux, uy);
-            cell_features(g, rho, ux, uy, s_in, xin);
+            if (learned) cell_features(g, rho, ux, uy, s_in, xin);
         }
         bf16x8 lo, hi;
 #pragma unroll
@@ -808,11 +887,14 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
         if (fluid) {
             float corr[9];
 #pragma unroll
-            for (int i = 0; i < 9; ++i) corr[i] = ys[wave][lane * kMlpOut + i] * (w_phys(i) * s_out);
-            project_correction(corr);
+            for (int i = 0; i < 9; ++i)
+                corr[i] = fminf(fmaxf(ys[wave][lane * kMlpOut + i], -kMlpOutClamp), kMlpOutClamp) * (w_phys(i) * s_out);
+            project_stress(corr);
             phys_collide<FORCE>(a, g, rho, ux, uy);
+            if (learned) {
 #pragma unroll
-            for (int i = 0; i < 9; ++i) g[i] += corr[i];
+                for (int i = 0; i < 9; ++i) g[i] += corr[i];
+            }
             cell_store<T, XEDGE, kNtStore>(a, dst, nb, x, y, g);
         }
     }
@@ -1367,6 +1449,23 @@ static hipError_t launch_phys_variant_shfl(const PhysArgs& a, const void* src, v
     }
 }
 
+template <typename T, int XEDGE, bool FORCE, int BW, int BH, int FLAGS>
+static hipError_t launch_phys_variant2(const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
+                                       uint32_t words, hipStream_t s)
+{
+    if constexpr (XEDGE != 0) {
+        return hipErrorInvalidValue;
+    } else {
+        // whole tiles only; the one-cell ring wraps, so nx, ny >= 3
+        if (a.nx % BW != 0 || a.ny % BH != 0 || a.nx < 3 || a.ny < 3) return hipErrorInvalidValue;
+        const uint32_t tiles_x = (uint32_t)(a.nx / BW);
+        const uint32_t tiles = tiles_x * (uint32_t)(a.ny / BH);
+        hipLaunchKernelGGL((k_phys_step2<T, FORCE, BW, BH, FLAGS>), dim3(tiles), dim3(256), 0, s, a, (const T*)src,
+                           (T*)dst, bits, words, tiles_x);
+        return hipGetLastError();
+    }
+}
+
 // Tile shapes the step kernel is built for: threads per block, cells per
 // thread, consecutive cells per lane (4 needs nx % 4 == 0) and the kNt* /
 // kXcdRemap flags.  Every shape computes the same bits.
@@ -1389,13 +1488,20 @@ const StepVariant kStepVariants[] = {
     {256, 1, 8, kNtStore | kShfl},
     {64, 1, 8, kNtStore | kShfl},
     {512, 1, 4, kNtStore | kShfl},
+    // two steps per launch: (block, tile rows, tile width, flags)
+    {256, 8, 128, kNtStore | kTwoStep},
+    {256, 16, 64, kNtStore | kTwoStep},
+    {256, 8, 256, kNtStore | kTwoStep},
+    {256, 4, 256, kNtStore | kTwoStep},
 };
 const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
 
-bool step_variant_ok(int v, int nx, int xedge)
+bool step_variant_ok(int v, int nx, int ny, int xedge)
 {
     if (v < 0 || v >= kNumStepVariants) return false;
     const StepVariant& sv = kStepVariants[v];
+    if (sv.flags & kTwoStep)
+        return xedge == 0 && nx % sv.vec == 0 && ny % sv.cells_per_thread == 0 && nx >= 3 && ny >= 3;
     if (sv.flags & kShfl) return xedge == 0 && nx % (sv.block * sv.vec) == 0;
     return sv.vec == 1 || nx % sv.vec == 0;
 }
@@ -1433,6 +1539,31 @@ static hipError_t dispatch_variant(int v, const PhysArgs& a, const void* src, vo
 #undef LBM_V
 #undef LBM_V4
 #undef LBM_VS
+}
+
+template <typename T, int XEDGE, bool FORCE>
+static hipError_t dispatch_variant2(int v, const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
+                                    uint32_t words, hipStream_t s)
+{
+    constexpr int S = kNtStore;
+    switch (v) {
+    case 18: return launch_phys_variant2<T, XEDGE, FORCE, 128, 8, S>(a, src, dst, bits, words, s);
+    case 19: return launch_phys_variant2<T, XEDGE, FORCE, 64, 16, S>(a, src, dst, bits, words, s);
+    case 20: return launch_phys_variant2<T, XEDGE, FORCE, 256, 8, S>(a, src, dst, bits, words, s);
+    case 21: return launch_phys_variant2<T, XEDGE, FORCE, 256, 4, S>(a, src, dst, bits, words, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_phys_step2(int variant, bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
+                             void* dst, const uint32_t* bits, uint32_t words, hipStream_t s)
+{
+    if (xedge) return hipErrorInvalidValue;
+    if (f16)
+        return force ? dispatch_variant2<_Float16, 0, true>(variant, a, src, dst, bits, words, s)
+                     : dispatch_variant2<_Float16, 0, false>(variant, a, src, dst, bits, words, s);
+    return force ? dispatch_variant2<float, 0, true>(variant, a, src, dst, bits, words, s)
+                 : dispatch_variant2<float, 0, false>(variant, a, src, dst, bits, words, s);
 }
 
 template <typename T, int XEDGE>

@@ -233,6 +233,26 @@ __global__ __launch_bounds__(256) void k_mlp_grad_reduce(const float* __restrict
     }
 }
 
+// Sum of squares of a[0..count) in binary64: grid-stride, one partial per
+// block (the host adds the partials in block order).
+__global__ __launch_bounds__(256) void k_sumsq(const float* __restrict__ a, size_t count, double* __restrict__ partial)
+{
+    double acc = 0.0;
+    for (size_t k = blockIdx.x * 256ull + threadIdx.x; k < count; k += (size_t)gridDim.x * 256ull)
+        acc += (double)a[k] * (double)a[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ __launch_bounds__(256) void k_scale(float* __restrict__ a, size_t count, float s)
+{
+    for (size_t k = blockIdx.x * 256ull + threadIdx.x; k < count; k += (size_t)gridDim.x * 256ull) a[k] *= s;
+}
+
 // Adam (Kingma & Ba 2015, Algorithm 1) on the binary32 master parameters.
 __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g,
                                                   float* __restrict__ m, float* __restrict__ v, float lr, float b1,
@@ -297,18 +317,11 @@ __global__ __launch_bounds__(256) void k_mlp_transition(const float* __restrict_
         const int tx = (x + ex_of(i) + nx) % nx, ty = (y + ey_of(i) + ny) % ny;
         d[i] = s1[(size_t)i * n + (uint32_t)ty * nx + tx] - bgk;
     }
-    float a = 0.0f, bx = 0.0f, by = 0.0f;
+    // the part of the residual the step can add: its second-order
+    // non-conserved moments (project_stress)
+    project_stress(d);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        a += d[i];
-        bx += (float)ex_of(i) * d[i];
-        by += (float)ey_of(i) * d[i];
-    }
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        g[i] = d[i] - w_phys(i) * (a + 3.0f * ((float)ex_of(i) * bx + (float)ey_of(i) * by));
-        g[i] /= (w_phys(i) * s_out);
-    }
+    for (int i = 0; i < 9; ++i) g[i] = d[i] / (w_phys(i) * s_out);
     const int slot = atomicAdd(count, 1);
     if (slot >= max_samples) return;
 #pragma unroll
@@ -570,6 +583,42 @@ int lbm_mlp_clear_samples(lbm_mlp* m)
 {
     if (!m) return lbm_fail(LBM_ERR_INVALID, "null network");
     m->samples = 0;
+    return LBM_OK;
+}
+
+int lbm_mlp_normalize_samples(lbm_mlp* m, float* in_rms, float* out_rms)
+{
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (m->samples < 1) return lbm_fail(LBM_ERR_INVALID, "no samples to normalize");
+    constexpr int kBlocks = 256;
+    double* part = nullptr;
+    MLP_TRY(hipMalloc(&part, 2 * kBlocks * sizeof(double)));
+    const size_t nx = (size_t)m->samples * kMlpIn, nt = (size_t)m->samples * kMlpOut;
+    hipLaunchKernelGGL(k_sumsq, dim3(kBlocks), dim3(256), 0, m->stream, m->X, nx, part);
+    hipLaunchKernelGGL(k_sumsq, dim3(kBlocks), dim3(256), 0, m->stream, m->T, nt, part + kBlocks);
+    double host[2 * kBlocks];
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host, part, sizeof host, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(part);
+    if (e != hipSuccess) return lbm_fail(LBM_ERR_HIP, "sample statistics: %s", hipGetErrorString(e));
+    double sx = 0.0, st = 0.0;
+    for (int b = 0; b < kBlocks; ++b) {
+        sx += host[b];
+        st += host[kBlocks + b];
+    }
+    const float rx = (float)std::sqrt(sx / (double)nx), rt = (float)std::sqrt(st / (double)nt);
+    if (!(rx > 0.0f) || !(rt > 0.0f) || !std::isfinite(rx) || !std::isfinite(rt))
+        return lbm_fail(LBM_ERR_INVALID, "samples have zero or non-finite spread");
+    hipLaunchKernelGGL(k_scale, dim3(kBlocks), dim3(256), 0, m->stream, m->X, nx, 1.0f / rx);
+    hipLaunchKernelGGL(k_scale, dim3(kBlocks), dim3(256), 0, m->stream, m->T, nt, 1.0f / rt);
+    MLP_TRY(hipGetLastError());
+    MLP_TRY(hipStreamSynchronize(m->stream));
+    m->s_in *= rx;
+    m->s_out *= rt;
+    if (in_rms) *in_rms = rx;
+    if (out_rms) *out_rms = rt;
     return LBM_OK;
 }
 

@@ -129,6 +129,7 @@ _SIGNATURES = [
     ("lbm_mlp_add_samples", C.c_int, [_P, _P, _P, C.c_int]),
     ("lbm_mlp_add_transition", C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                          C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    ("lbm_mlp_normalize_samples", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("lbm_mlp_train", C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_uint64, C.POINTER(C.c_double)]),
     ("lbm_set_learned_collision", C.c_int, [_P, _P]),
     ("lbm_set_learned_precision", C.c_int, [_P, C.c_int]),
@@ -194,7 +195,9 @@ def device_count() -> int:
 def variant_shapes() -> list[tuple[int, int, int, int]]:
     """(threads per block, cells per thread, consecutive cells per lane,
     flags: 1 non-temporal stores, 2 non-temporal loads, 4 rows per XCD,
-    8 aligned vector loads with x-neighbours from the adjacent lane)."""
+    8 aligned vector loads with x-neighbours from the adjacent lane, 16 two
+    steps per launch through an LDS tile of cells_per_thread rows x vec
+    cells)."""
     lib = load_library()
     out = []
     for v in range(lib.lbm_variant_count()):
@@ -204,11 +207,15 @@ def variant_shapes() -> list[tuple[int, int, int, int]]:
     return out
 
 
-def variant_ok(variant: int, nx: int, x_edge: int = X_PERIODIC) -> bool:
-    """Whether a tile variant can step an nx-wide lattice (the library's
-    own rule: lane-shuffle variants need periodic x and whole blocks of
-    block*vec cells per row; vector variants need nx % vec == 0)."""
-    b, _, v, f = variant_shapes()[variant]
+def variant_ok(variant: int, nx: int, ny: int, x_edge: int = X_PERIODIC) -> bool:
+    """Whether a tile variant can step an nx x ny lattice (the library's
+    own rule: two-step variants need periodic x and whole tiles of
+    (rows, width) = (cells_per_thread, vec); lane-shuffle variants need
+    periodic x and whole blocks of block*vec cells per row; vector variants
+    need nx % vec == 0)."""
+    b, c, v, f = variant_shapes()[variant]
+    if f & 16:
+        return x_edge == X_PERIODIC and nx % v == 0 and ny % c == 0 and nx >= 3 and ny >= 3
     if f & 8:
         return x_edge == X_PERIODIC and nx % (b * v) == 0
     return v == 1 or nx % v == 0
@@ -573,6 +580,12 @@ class Network:
         _check(self._lib.lbm_mlp_add_transition(self._h, _ptr(s0), _ptr(s1), _ptr(sd), nx, ny, tau, force[0],
                                                  force[1], stride, offset, C.byref(added)))
         return added.value
+
+    def normalize_samples(self) -> tuple[float, float]:
+        """Unit-rms stored samples; the factors fold into the scales."""
+        rx, rt = C.c_float(), C.c_float()
+        _check(self._lib.lbm_mlp_normalize_samples(self._h, C.byref(rx), C.byref(rt)))
+        return rx.value, rt.value
 
     def train(self, steps: int, batch: int, lr: float, seed: int = 7) -> float:
         loss = C.c_double()

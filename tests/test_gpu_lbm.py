@@ -263,12 +263,15 @@ def test_physics_reduce_identities_small():
 
 
 @pytest.mark.parametrize("storage", [L.STORAGE_F32, L.STORAGE_F16_DEV], ids=["f32", "f16"])
-@pytest.mark.parametrize("nx, ny", [(300, 200), (2048, 40)])
+@pytest.mark.parametrize("nx, ny", [(300, 200), (2048, 40), (512, 64)])
 def test_tile_variants_and_graphs_bit_identical(storage, nx, ny):
     """Every tile variant that can step the lattice, with and without
     graphs, in both device layouts (row pitch 304 > nx = 300 in the
     row-interleaved one) gives the same populations bit for bit; the
-    2048-wide lattice admits the lane-shuffle variants."""
+    2048-wide lattice admits the lane-shuffle variants and the 512 x 64 one
+    the two-step (LDS tile) variants.  202 + 1 steps with graph chunks of 0,
+    100 or 102 steps (51 two-step launches: an odd number of buffer flips
+    per graph) and a final single step (the one-step fallback)."""
     print(f"case: variants + graphs {nx}x{ny} storage {storage}", flush=True)
     scene = W.periodic_disks(nx, ny, 0.05, 4, 10, seed=4)
     f0 = W.rest_state(nx, ny, scene.solid, amplitude=1e-3).astype(np.float32)
@@ -276,19 +279,22 @@ def test_tile_variants_and_graphs_bit_identical(storage, nx, ny):
     nv = len(L.variant_shapes())
     for layout in (L.LAYOUT_ROWS, L.LAYOUT_PLANES):
         for v in range(nv):
-            if not L.variant_ok(v, nx):
+            if not L.variant_ok(v, nx, ny):
                 continue
             lat = L.Lattice(nx, ny, 0.58, force=(1e-5, 0.0), layout=layout, storage=storage)
             lat.write_materials(W.materials_from_solid(scene.solid))
             lat.write_populations(f0, 0)
             lat.set_variant(v)
-            lat.set_graph_chunk(0 if v % 2 else 100)
+            lat.set_graph_chunk((0, 100, 102)[v % 3])
             lat.run(202)
+            lat.run(1)
             outs.append(lat.read_populations())
             names.append((layout, v, L.variant_shapes()[v]))
             lat.close()
     if nx % 1024 == 0:
         assert any(n[2][3] & 8 for n in names), "no lane-shuffle variant ran"
+    if nx == 512:
+        assert sum(1 for n in names if n[2][3] & 16) >= 6, "two-step variants did not run"
     for k, o in enumerate(outs[1:], 1):
         assert np.array_equal(o, outs[0]), f"variant {names[k]} differs from {names[0]}"
 
