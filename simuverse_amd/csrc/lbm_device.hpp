@@ -66,7 +66,8 @@ enum : int {
     kNtLoad = 2,   // non-temporal population loads
     kXcdRemap = 4, // give each XCD a contiguous range of rows
     kShfl = 8,     // aligned vector loads, x-neighbours from the adjacent lane
-    kTwoStep = 16  // two steps per launch through an LDS tile (temporal blocking)
+    kTwoStep = 16, // two steps per launch through an LDS tile (temporal blocking)
+    kGridStride = 32  // batched: block b takes chunks b, b + grid, ... (else a contiguous range)
 };
 
 // Population storage of the physics mode.  Kernels work on deviations

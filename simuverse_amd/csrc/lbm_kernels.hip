@@ -683,7 +683,9 @@ __device__ inline void batch_chunk(const BatchDomainDev& D, uint32_t t, const T*
     }
 }
 
-// Block b (after the XCD remap) steps chunks [b*per_block, (b+1)*per_block).
+// Block b (after the XCD remap) steps chunks [b*per_block, (b+1)*per_block),
+// or, with kGridStride, chunks b, b + grid, b + 2 grid, ... so that the
+// blocks running at one time work on neighbouring chunks.
 template <typename T, int BS, int V, int FLAGS>
 __global__ __launch_bounds__(BS) void k_batch_step(const BatchDomainDev* __restrict__ doms,
                                                    const uint16_t* __restrict__ chunk_dom, uint32_t nchunks,
@@ -691,9 +693,11 @@ __global__ __launch_bounds__(BS) void k_batch_step(const BatchDomainDev* __restr
                                                    const uint32_t* __restrict__ bits)
 {
     const uint32_t b = (FLAGS & kXcdRemap) ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-    const uint32_t t0 = b * per_block;
-    const uint32_t t1 = min(t0 + per_block, nchunks);
-    for (uint32_t t = t0; t < t1; ++t) {
+    const bool strided = (FLAGS & kGridStride) != 0;
+    const uint32_t t0 = strided ? b : b * per_block;
+    const uint32_t t1 = strided ? nchunks : min(t0 + per_block, nchunks);
+    const uint32_t dt = strided ? gridDim.x : 1u;
+    for (uint32_t t = t0; t < t1; t += dt) {
         const BatchDomainDev D = doms[chunk_dom[t]];
         if (D.x_edge) {
             if (D.force) batch_chunk<T, 1, true, BS, V, FLAGS>(D, t, src, dst, bits);
@@ -1593,6 +1597,8 @@ hipError_t launch_phys_step(int variant, bool f16, int xedge, bool force, bool e
 const BatchVariant kBatchVariants[] = {
     {256, 1, 8, kNtStore}, {256, 1, 8, kNtStore | kXcdRemap}, {512, 1, 4, kNtStore}, {256, 1, 8, 0},
     {64, 4, 16, kNtStore}, {128, 4, 8, 0},
+    {256, 1, 8, kNtStore | kGridStride}, {512, 1, 4, kNtStore | kGridStride}, {1024, 1, 2, kNtStore | kGridStride},
+    {256, 1, 16, kNtStore | kGridStride},
 };
 const int kNumBatchVariants = (int)(sizeof(kBatchVariants) / sizeof(kBatchVariants[0]));
 
@@ -1611,6 +1617,10 @@ static hipError_t launch_batch_typed(int v, const BatchDomainDev* doms, const ui
     case 3: LBM_BATCH(256, 1, 0); break;
     case 4: LBM_BATCH(64, 4, kNtStore); break;
     case 5: LBM_BATCH(128, 4, 0); break;
+    case 6: LBM_BATCH(256, 1, kNtStore | kGridStride); break;
+    case 7: LBM_BATCH(512, 1, kNtStore | kGridStride); break;
+    case 8: LBM_BATCH(1024, 1, kNtStore | kGridStride); break;
+    case 9: LBM_BATCH(256, 1, kNtStore | kGridStride); break;
     default: return hipErrorInvalidValue;
     }
 #undef LBM_BATCH
