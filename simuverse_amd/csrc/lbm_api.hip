@@ -109,6 +109,7 @@ struct lbm_handle {
         a.inlet_ux = d.inlet_ux;
         a.scale = scale;
         a.inv_scale = inv_scale;
+        set_force_terms(a);
         return a;
     }
     WgslArgs wgsl() const

@@ -245,6 +245,7 @@ int lbm_batch_create(const lbm_domain_desc* domains, int count, int storage, lbm
         D.a.inlet_ux = q.inlet_ux;
         D.a.scale = b->scale;
         D.a.inv_scale = b->inv_scale;
+        set_force_terms(D.a);
         D.pop_off = (uint32_t)pop;
         D.bits_off = (uint32_t)word;
         D.words = solid_mask_words(q.nx);

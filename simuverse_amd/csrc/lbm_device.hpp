@@ -137,7 +137,29 @@ struct PhysArgs {
     float inlet_ux;   // Zou-He inlet velocity (x_edge == 1)
     float scale;      // binary16 deviation scale 2^shift
     float inv_scale;  // 2^-shift
+    // Guo forcing F_i = fcoef w_i (3 (e_i - u).g + 9 (e_i.u)(e_i.g))
+    //                 = fa_i + fb_i (e_i.u) - fc_k (u.g)   (k: weight class)
+    float fa[9];      // 3 fcoef w_i (e_i.g)
+    float fb[9];      // 9 fcoef w_i (e_i.g)
+    float fc[3];      // 3 fcoef w_k for the rest, axis and diagonal weights
 };
+
+// Fills fcoef and the Guo coefficient tables from omega, gx, gy.
+__host__ __device__ inline void set_force_terms(PhysArgs& a)
+{
+    a.fcoef = 1.0f - 0.5f * a.omega;
+    for (int i = 0; i < 9; ++i) {
+        const float eg = (float)ex_of(i) * a.gx + (float)ey_of(i) * a.gy;
+        a.fa[i] = 3.0f * a.fcoef * w_phys(i) * eg;
+        a.fb[i] = 9.0f * a.fcoef * w_phys(i) * eg;
+    }
+    a.fc[0] = 3.0f * a.fcoef * w_phys(0);
+    a.fc[1] = 3.0f * a.fcoef * w_phys(1);
+    a.fc[2] = 3.0f * a.fcoef * w_phys(5);
+}
+
+// Weight class of direction i: 0 rest, 1 axis, 2 diagonal.
+__host__ __device__ constexpr int wclass(int i) { return i == 0 ? 0 : (i <= 4 ? 1 : 2); }
 
 __host__ __device__ inline uint32_t pop_index(const PhysArgs& a, int x, int y, int i)
 {
@@ -157,6 +179,67 @@ struct BatchDomainDev {
     int32_t force;      // non-zero when (gx, gy) != 0
     uint32_t pad;
 };
+
+// Moments of the post-streaming deviations (Guo velocity with FORCE).
+template <bool FORCE>
+__device__ inline void phys_moments(const PhysArgs& a, const float g[9], float& rho, float& ux, float& uy)
+{
+    float dr = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dr += g[i];
+    const float jx = (g[1] - g[3]) + (g[5] - g[6]) + (g[8] - g[7]);
+    const float jy = (g[4] - g[2]) + (g[7] - g[5]) + (g[8] - g[6]);
+    rho = 1.0f + dr;
+    const float ir = __builtin_amdgcn_rcpf(rho);  // hardware reciprocal, 1 ulp
+    if (FORCE) {
+        ux = (jx + 0.5f * a.gx) * ir;
+        uy = (jy + 0.5f * a.gy) * ir;
+    } else {
+        ux = jx * ir;
+        uy = jy * ir;
+    }
+}
+
+// BGK (+ Guo forcing) in deviation form: g*_i = g_i - omega (g_i - geq_i) + F_i
+// with geq_i = feq_i - w_i = w_i (rho - 1) + w_i rho (e.u (3 + 4.5 e.u) - 1.5 u^2)
+// and F_i = fa_i + fb_i (e_i.u) - fc_k (u.g) (PhysArgs): nine fused
+// multiply-adds per direction, the per-cell products w_k rho, w_k (rho - 1)
+// and u.g shared by all directions of a weight class.
+template <bool FORCE>
+__device__ inline void phys_collide(const PhysArgs& a, float g[9], float rho, float ux, float uy)
+{
+    const float dr = rho - 1.0f;
+    const float usq = 1.5f * fmaf(ux, ux, uy * uy);
+    const float wr[3] = {w_phys(0) * rho, w_phys(1) * rho, w_phys(5) * rho};
+    const float wd[3] = {w_phys(0) * dr, w_phys(1) * dr, w_phys(5) * dr};
+    const float ug = FORCE ? fmaf(ux, a.gx, uy * a.gy) : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int ex = ex_of(i), ey = ey_of(i), k = wclass(i);
+        const float eu = ex == 0 ? (ey == 0 ? 0.0f : (ey > 0 ? uy : -uy))
+                                 : (ey == 0 ? (ex > 0 ? ux : -ux)
+                                            : (ex > 0 ? (ey > 0 ? ux + uy : ux - uy) : (ey > 0 ? uy - ux : -ux - uy)));
+        const float q = fmaf(eu, fmaf(4.5f, eu, 3.0f), -usq);
+        const float geq = fmaf(wr[k], q, wd[k]);
+        float out = fmaf(-a.omega, g[i] - geq, g[i]);
+        if (FORCE) out += fmaf(a.fb[i], eu, fmaf(-a.fc[k], ug, a.fa[i]));
+        g[i] = out;
+    }
+}
+
+// The learned correction's input: the non-equilibrium part g - geq = f - feq
+// over w_i s_in.
+__device__ __forceinline__ void cell_features(const float g[9], float rho, float ux, float uy, float s_in,
+                                              float xin[9])
+{
+    const float dr = rho - 1.0f, usq = 1.5f * (ux * ux + uy * uy);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float eu = (float)ex_of(i) * ux + (float)ey_of(i) * uy;
+        const float geq = w_phys(i) * (dr + rho * (3.0f * eu + 4.5f * (eu * eu) - usq));
+        xin[i] = (g[i] - geq) / (w_phys(i) * s_in);
+    }
+}
 
 // Arguments of the parity-mode (reference semantics) kernels.
 struct WgslArgs {

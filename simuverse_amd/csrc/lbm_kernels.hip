@@ -71,47 +71,6 @@ __device__ inline void phys_gather(const PhysArgs& a, const T* __restrict__ src,
     }
 }
 
-template <bool FORCE>
-__device__ inline void phys_moments(const PhysArgs& a, const float g[9], float& rho, float& ux, float& uy)
-{
-    float dr = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) dr += g[i];
-    const float jx = (g[1] - g[3]) + (g[5] - g[6]) + (g[8] - g[7]);
-    const float jy = (g[4] - g[2]) + (g[7] - g[5]) + (g[8] - g[6]);
-    rho = 1.0f + dr;
-    const float ir = 1.0f / rho;
-    if (FORCE) {
-        ux = (jx + 0.5f * a.gx) * ir;
-        uy = (jy + 0.5f * a.gy) * ir;
-    } else {
-        ux = jx * ir;
-        uy = jy * ir;
-    }
-}
-
-// BGK (+ Guo forcing) in deviation form: g*_i = g_i - omega (g_i - geq_i) + F_i
-// with geq_i = feq_i - w_i = w_i ((rho - 1) + rho (3 eu + 4.5 eu^2 - 1.5 u^2)).
-template <bool FORCE>
-__device__ inline void phys_collide(const PhysArgs& a, float g[9], float rho, float ux, float uy)
-{
-    const float dr = rho - 1.0f;
-    const float usq = 1.5f * (ux * ux + uy * uy);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const float exf = (float)ex_of(i), eyf = (float)ey_of(i);
-        const float eu = exf * ux + eyf * uy;
-        const float geq = w_phys(i) * (dr + rho * (3.0f * eu + 4.5f * (eu * eu) - usq));
-        float out = g[i] - a.omega * (g[i] - geq);
-        if (FORCE) {
-            const float fx = 3.0f * (exf - ux) + 9.0f * eu * exf;
-            const float fy = 3.0f * (eyf - uy) + 9.0f * eu * eyf;
-            out += a.fcoef * w_phys(i) * (fx * a.gx + fy * a.gy);
-        }
-        g[i] = out;
-    }
-}
-
 // Packed solid mask: row y holds bit x+1 for cell x; bit 0 and bit nx+1
 // are the x-neighbours beyond the edges (the wrapped cells when x is
 // periodic, empty otherwise).  `words` per row is even and covers bit
@@ -211,20 +170,6 @@ __device__ __forceinline__ void cell_store(const PhysArgs& a, T* __restrict__ ds
         const int tx = ex == 1 ? xp : (ex == -1 ? xm : x);
         const uint32_t trow = ey == 1 ? rp : (ey == -1 ? rm : r0);
         Store<T>::store_dev(dst, trow + (uint32_t)tx + (uint32_t)inv_of(i) * a.plane, g[i], w_phys(i), a.scale);
-    }
-}
-
-// The learned correction's input: the non-equilibrium part g - geq = f - feq
-// over w_i s_in.
-__device__ __forceinline__ void cell_features(const float g[9], float rho, float ux, float uy, float s_in,
-                                              float xin[9])
-{
-    const float dr = rho - 1.0f, usq = 1.5f * (ux * ux + uy * uy);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const float eu = (float)ex_of(i) * ux + (float)ey_of(i) * uy;
-        const float geq = w_phys(i) * (dr + rho * (3.0f * eu + 4.5f * (eu * eu) - usq));
-        xin[i] = (g[i] - geq) / (w_phys(i) * s_in);
     }
 }
 

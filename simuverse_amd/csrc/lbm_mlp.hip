@@ -277,11 +277,12 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ p, const f
 // solid or have a solid neighbour are skipped; y and x are periodic.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_mlp_transition(const float* __restrict__ s0, const float* __restrict__ s1,
-                                                        const uint8_t* __restrict__ solid, int nx, int ny, float omega,
-                                                        float gx, float gy, float s_in, float s_out, uint32_t stride,
-                                                        uint32_t offset, int max_samples, int* __restrict__ count,
+                                                        const uint8_t* __restrict__ solid, PhysArgs a, float s_in,
+                                                        float s_out, uint32_t stride, uint32_t offset,
+                                                        int max_samples, int* __restrict__ count,
                                                         float* __restrict__ X, float* __restrict__ T)
 {
+    const int nx = a.nx, ny = a.ny;
     const uint32_t n = (uint32_t)nx * (uint32_t)ny;
     const uint32_t k = blockIdx.x * 256u + threadIdx.x;
     const uint64_t c64 = (uint64_t)k * stride + offset;
@@ -293,41 +294,32 @@ __global__ __launch_bounds__(256) void k_mlp_transition(const float* __restrict_
             const int xx = (x + dx2 + nx) % nx, yy = (y + dy2 + ny) % ny;
             if (solid[(uint32_t)yy * nx + xx]) return;
         }
-    float f[9], g[9];
-    float rho = 0.0f, jx = 0.0f, jy = 0.0f;
+    // The step's own arithmetic on deviations g = f - w: moments, features,
+    // BGK (+ Guo).
+    float g[9], gc[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        f[i] = s0[(size_t)i * n + c];
-        rho += f[i];
-        jx += (float)ex_of(i) * f[i];
-        jy += (float)ey_of(i) * f[i];
-    }
-    const float ux = (jx + 0.5f * gx) / rho, uy = (jy + 0.5f * gy) / rho;
-    const float usq = 1.5f * (ux * ux + uy * uy);
-    const float fcoef = 1.0f - 0.5f * omega;
+    for (int i = 0; i < 9; ++i) g[i] = s0[(size_t)i * n + c] - w_phys(i);
+    float rho, ux, uy;
+    phys_moments<true>(a, g, rho, ux, uy);
     float xin[9], d[9];
+    cell_features(g, rho, ux, uy, s_in, xin);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) gc[i] = g[i];
+    phys_collide<true>(a, gc, rho, ux, uy);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        const float exf = (float)ex_of(i), eyf = (float)ey_of(i);
-        const float eu = exf * ux + eyf * uy;
-        const float feq = w_phys(i) * rho * (1.0f + 3.0f * eu + 4.5f * eu * eu - usq);
-        xin[i] = (f[i] - feq) / (w_phys(i) * s_in);
-        const float fx = 3.0f * (exf - ux) + 9.0f * eu * exf, fy = 3.0f * (eyf - uy) + 9.0f * eu * eyf;
-        const float bgk = f[i] - omega * (f[i] - feq) + fcoef * w_phys(i) * (fx * gx + fy * gy);
         const int tx = (x + ex_of(i) + nx) % nx, ty = (y + ey_of(i) + ny) % ny;
-        d[i] = s1[(size_t)i * n + (uint32_t)ty * nx + tx] - bgk;
+        d[i] = (s1[(size_t)i * n + (uint32_t)ty * nx + tx] - w_phys(i)) - gc[i];
     }
     // the part of the residual the step can add: its second-order
     // non-conserved moments (project_stress)
     project_stress(d);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) g[i] = d[i] / (w_phys(i) * s_out);
     const int slot = atomicAdd(count, 1);
     if (slot >= max_samples) return;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         X[(size_t)slot * 9 + i] = xin[i];
-        T[(size_t)slot * 9 + i] = g[i];
+        T[(size_t)slot * 9 + i] = d[i] / (w_phys(i) * s_out);
     }
 }
 
@@ -663,9 +655,17 @@ int lbm_mlp_add_transition(lbm_mlp* m, const float* state_t, const float* state_
     if (e == hipSuccess) e = hipMemcpyAsync(m->count, &m->samples, sizeof(int), hipMemcpyHostToDevice, m->stream);
     const size_t cand = (n - (size_t)std::min<int>(offset, (int)n) + stride - 1) / stride;
     if (e == hipSuccess && cand > 0) {
+        PhysArgs pa{};
+        pa.nx = nx;
+        pa.ny = ny;
+        pa.omega = 1.0f / tau;
+        pa.gx = force_x;
+        pa.gy = force_y;
+        pa.scale = pa.inv_scale = 1.0f;
+        set_force_terms(pa);
         hipLaunchKernelGGL(k_mlp_transition, dim3((unsigned)((cand + 255) / 256)), dim3(256), 0, m->stream, s0, s1, sd,
-                           nx, ny, 1.0f / tau, force_x, force_y, m->s_in, m->s_out, (uint32_t)stride, (uint32_t)offset,
-                           m->capacity, m->count, m->X, m->T);
+                           pa, m->s_in, m->s_out, (uint32_t)stride, (uint32_t)offset, m->capacity, m->count, m->X,
+                           m->T);
         e = hipGetLastError();
     }
     int total = m->samples;
