@@ -296,13 +296,15 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* best_variant, double* ms_pe
 /* ---- learned collision correction --------------------------------------
  * No reference counterpart.  A per-cell network 9 -> 64 -> 64 -> 9 (GELU,
  * erf form) maps the non-equilibrium part of the gathered populations,
- * x_i = (f_i - feq_i) / (w_i * in_scale), to a correction
- * df_i = w_i * out_scale * clamp(y_i, +-8); the correction is projected
- * onto the second-order non-conserved moments (energy e and deviatoric
- * stresses p_xx, p_xy of the orthogonal moment basis: no mass, no momentum,
- * no ghost moments) and added to the BGK post-collision populations of
- * cells whose 3x3 neighbourhood is fluid (the cells the network is trained
- * on).  Parameters: one vector of 5385 floats,
+ * x_i = (f_i - feq_i) / (w_i * in_scale), to y; its outputs y_0..y_2
+ * modulate the non-equilibrium parts n_a of the second-order non-conserved
+ * moments a = e, p_xx, p_xy (orthogonal moment basis) by alpha_a * n_a,
+ * alpha_a = clamp(out_scale * y_a, -(2 - omega)/2, 0.5) (a bound that keeps
+ * the relaxation stable), mapped back to populations (no mass, no
+ * momentum, no ghost moments) and added to the BGK post-collision
+ * populations of cells whose 3x3 neighbourhood is fluid (the cells the
+ * network is trained on).  Outputs 3..8 are unused (trained to zero).
+ * Parameters: one vector of 5385 floats,
  * W1[64][9] b1[64] W2[64][64] b2[64] W3[9][64] b3[9] (row = output unit).
  * Training (forward, backward, Adam) runs on the device in binary32. */
 typedef struct lbm_mlp lbm_mlp;
@@ -330,8 +332,10 @@ int lbm_mlp_add_samples(lbm_mlp* m, const float* x, const float* t, int n);
 /* Samples from two consecutive pre-collision states of a periodic lattice
  * (dense f[i*nx*ny + c], binary32): for every stride-th cell from offset
  * whose 3x3 neighbourhood is fluid, x = the network input at time t and
- * t = the residual f_i(c + e_i, t+1) - BGK_i(f(c, t)) projected like the
- * correction, in output units.  Uses the current scales. */
+ * t_a = clamp(d_a n_a / (n_a^2 + eps^2)) / out_scale for a = e, p_xx, p_xy (0 for
+ * outputs 3..8), d_a the moments of the residual f_i(c + e_i, t+1) -
+ * BGK_i(f(c, t)), eps^2 = 1e-2 (n_xx^2 + n_xy^2 + n_e^2 / 9): the
+ * modulation that reproduces the residual.  Uses the current scales. */
 int lbm_mlp_add_transition(lbm_mlp* m, const float* state_t, const float* state_t1, const uint8_t* solid, int nx,
                            int ny, float tau, float force_x, float force_y, int stride, int offset, int* added);
 /* Rescales the stored samples to unit root-mean-square (one factor for all

@@ -259,10 +259,6 @@ constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH;
 constexpr int kOffW3 = kOffB2 + kMlpH;
 constexpr int kOffB3 = kOffW3 + kMlpOut * kMlpH;
 constexpr int kMlpParams = kOffB3 + kMlpOut;  // 5385
-// The step clamps network outputs to +-kMlpOutClamp: training targets are
-// normalised to unit root-mean-square, so this bounds a correction to a few
-// times the largest the network was trained to produce.
-constexpr float kMlpOutClamp = 8.0f;
 
 __device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
 __device__ inline float gelu_grad(float x)
@@ -479,6 +475,46 @@ __device__ inline void project_stress(float d[9])
     for (int i = 0; i < 9; ++i) {
         const float ex = (float)ex_of(i), ey = (float)ey_of(i);
         d[i] = mom_e(i) * me + (ex * ex - ey * ey) * mxx + (ex * ey) * mxy;
+    }
+}
+
+// Bounds of the learned modulation alpha of a moment's non-equilibrium part
+// (learned_correction): BGK leaves (1 - omega) n_a after the collision and
+// the correction adds alpha n_a, so |1 - omega + alpha| < 1 needs
+// alpha > omega - 2; the lower bound keeps half of that margin.
+__host__ __device__ inline float modulation_lo(float omega) { return -0.5f * (2.0f - omega); }
+constexpr float kModulationHi = 0.5f;
+
+// The learned collision correction of one cell.  The network's outputs
+// 0, 1, 2 modulate the non-equilibrium parts n_a of the second-order
+// non-conserved moments a = e, p_xx, p_xy of the cell (a learned,
+// state-dependent change of how far BGK relaxes each of them):
+//   dm_a = alpha_a n_a,   alpha_a = clamp(s_out y_a, modulation_lo, 0.5),
+//   n_a = sum_i m_a(i) (f_i - feq_i),
+// mapped back with the orthogonal basis (project_stress):
+//   corr_i = sum_a m_a(i) dm_a / |m_a|^2,
+// so the correction carries no mass, no momentum and no ghost moments.
+// xin holds the network inputs (f_i - feq_i) / (w_i s_in).
+__device__ inline void learned_correction(const float xin[9], const float y[9], float s_in, float s_out, float omega,
+                                          float corr[9])
+{
+    float ne = 0.0f, nxx = 0.0f, nxy = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float ex = (float)ex_of(i), ey = (float)ey_of(i);
+        const float q = xin[i] * (w_phys(i) * s_in);
+        ne += mom_e(i) * q;
+        nxx += (ex * ex - ey * ey) * q;
+        nxy += (ex * ey) * q;
+    }
+    const float lo = modulation_lo(omega);
+    const float ce = fminf(fmaxf(s_out * y[0], lo), kModulationHi) * ne * (1.0f / 36.0f);
+    const float cxx = fminf(fmaxf(s_out * y[1], lo), kModulationHi) * nxx * 0.25f;
+    const float cxy = fminf(fmaxf(s_out * y[2], lo), kModulationHi) * nxy * 0.25f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float ex = (float)ex_of(i), ey = (float)ey_of(i);
+        corr[i] = mom_e(i) * ce + (ex * ex - ey * ey) * cxx + (ex * ey) * cxy;
     }
 }
 

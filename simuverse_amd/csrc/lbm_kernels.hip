@@ -209,12 +209,10 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
         // (away from walls and open edges); only those get the correction.
         learned = (nb.sm | nb.s0 | nb.sp) == 0u && (XEDGE == 0 || (x > 0 && x < a.nx - 1));
         if (learned) {
-            float xin[9];
+            float xin[9], y9[9];
             cell_features(g, rho, ux, uy, s_in, xin);
-            mlp_forward_one(mlp, xin, corr);
-#pragma unroll
-            for (int i = 0; i < 9; ++i) corr[i] = fminf(fmaxf(corr[i], -kMlpOutClamp), kMlpOutClamp) * (w_phys(i) * s_out);
-            project_stress(corr);
+            mlp_forward_one(mlp, xin, y9);
+            learned_correction(xin, y9, s_in, s_out, a.omega, corr);
         }
     }
     phys_collide<FORCE>(a, g, rho, ux, uy);
@@ -834,11 +832,10 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
         mlp_wave_bf16(W, bias, xs[wave], ys[wave], lane);
         __syncthreads();
         if (fluid) {
-            float corr[9];
+            float corr[9], y9[9];
 #pragma unroll
-            for (int i = 0; i < 9; ++i)
-                corr[i] = fminf(fmaxf(ys[wave][lane * kMlpOut + i], -kMlpOutClamp), kMlpOutClamp) * (w_phys(i) * s_out);
-            project_stress(corr);
+            for (int i = 0; i < 9; ++i) y9[i] = ys[wave][lane * kMlpOut + i];
+            learned_correction(xin, y9, s_in, s_out, a.omega, corr);
             phys_collide<FORCE>(a, g, rho, ux, uy);
             if (learned) {
 #pragma unroll

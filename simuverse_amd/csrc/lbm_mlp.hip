@@ -311,15 +311,33 @@ __global__ __launch_bounds__(256) void k_mlp_transition(const float* __restrict_
         const int tx = (x + ex_of(i) + nx) % nx, ty = (y + ey_of(i) + ny) % ny;
         d[i] = (s1[(size_t)i * n + (uint32_t)ty * nx + tx] - w_phys(i)) - gc[i];
     }
-    // the part of the residual the step can add: its second-order
-    // non-conserved moments (project_stress)
-    project_stress(d);
+    // Targets of the three modulations (learned_correction): residual
+    // moment over non-equilibrium moment, d_a / n_a, regularised where n_a
+    // is small against the cell's stress: d_a n_a / (n_a^2 + eps^2) with
+    // eps^2 = 1e-2 (n_xx^2 + n_xy^2 + n_e^2 / 9), clamped to the bounds the
+    // step applies.
+    float ne = 0.0f, nxx = 0.0f, nxy = 0.0f, de = 0.0f, dxx = 0.0f, dxy = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const float ex = (float)ex_of(i), ey = (float)ey_of(i);
+        const float q = xin[i] * (w_phys(i) * s_in);
+        ne += mom_e(i) * q;
+        nxx += (ex * ex - ey * ey) * q;
+        nxy += (ex * ey) * q;
+        de += mom_e(i) * d[i];
+        dxx += (ex * ex - ey * ey) * d[i];
+        dxy += (ex * ey) * d[i];
+    }
+    const float eps2 = 1e-2f * (nxx * nxx + nxy * nxy + ne * ne * (1.0f / 9.0f)) + 1e-30f;
+    const float lo = modulation_lo(a.omega);
+    auto ratio = [&](float dm, float nm) { return fminf(fmaxf(dm * nm / (nm * nm + eps2), lo), kModulationHi); };
+    float t[9] = {ratio(de, ne), ratio(dxx, nxx), ratio(dxy, nxy), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     const int slot = atomicAdd(count, 1);
     if (slot >= max_samples) return;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         X[(size_t)slot * 9 + i] = xin[i];
-        T[(size_t)slot * 9 + i] = d[i] / (w_phys(i) * s_out);
+        T[(size_t)slot * 9 + i] = t[i] / s_out;
     }
 }
 

@@ -271,7 +271,10 @@ def test_tile_variants_and_graphs_bit_identical(storage, nx, ny):
     2048-wide lattice admits the lane-shuffle variants and the 512 x 64 one
     the two-step (LDS tile) variants.  202 + 1 steps with graph chunks of 0,
     100 or 102 steps (51 two-step launches: an odd number of buffer flips
-    per graph) and a final single step (the one-step fallback)."""
+    per graph) and a final single step (the one-step fallback).  Compared at
+    every slot a fluid cell can read: a two-step launch ends in the other
+    buffer, whose solid-cell slots that no fluid cell pulls from hold older
+    values."""
     print(f"case: variants + graphs {nx}x{ny} storage {storage}", flush=True)
     scene = W.periodic_disks(nx, ny, 0.05, 4, 10, seed=4)
     f0 = W.rest_state(nx, ny, scene.solid, amplitude=1e-3).astype(np.float32)
@@ -295,8 +298,10 @@ def test_tile_variants_and_graphs_bit_identical(storage, nx, ny):
         assert any(n[2][3] & 8 for n in names), "no lane-shuffle variant ran"
     if nx == 512:
         assert sum(1 for n in names if n[2][3] & 16) >= 6, "two-step variants did not run"
+    vis = visible_mask(scene.solid)
     for k, o in enumerate(outs[1:], 1):
-        assert np.array_equal(o, outs[0]), f"variant {names[k]} differs from {names[0]}"
+        bad = np.count_nonzero(o[vis] != outs[0][vis])
+        assert bad == 0, f"variant {names[k]} differs from {names[0]} at {bad} visible slots"
 
 
 def test_checkpoint_resume_bit_identical():

@@ -7,8 +7,8 @@
 * training (device forward/backward/Adam) reduces the loss;
 * the projected correction leaves mass and momentum exactly as plain BGK
   does;
-* a network whose output blows up makes lbm_run_guarded fall back to plain
-  BGK, and the result equals the plain run bit for bit;
+* a chunk that leaves the healthy range makes lbm_run_guarded fall back to
+  plain BGK, and the result equals the plain run bit for bit;
 * the 4x restriction keeps block mass and momentum;
 * on a coarse lattice the trained network brings the velocity closer to the
   restricted fine solution than plain BGK does.
@@ -138,7 +138,7 @@ def test_learned_step_bf16_tracks_f32():
     f0 = W.rest_state(nx, ny, scene.solid, amplitude=1e-3, seed=4).astype(np.float32)
     net = L.Network(seed=6)
     net.set_params(random_params(10, 0.2))
-    net.set_scales(1e-3, 1e-4)
+    net.set_scales(1e-3, 0.05)
     out = {}
     for name, prec in (("plain", None), ("f32", L.MLP_F32), ("bf16", L.MLP_BF16)):
         lat = L.Lattice(nx, ny, 0.6, force=(1e-5, 0.0))
@@ -162,14 +162,15 @@ def test_blow_up_falls_back_to_bgk():
     solid = W.cylinder_channel_solid(nx, ny, cx=20, cy=24, r=5)
     f0 = W.rest_state(nx, ny, solid, amplitude=1e-4).astype(np.float32)
     net = L.Network(seed=8)
-    net.set_params(random_params(9, 0.3) * 1e3)  # a network that explodes
-    # outputs clamp at +-8: corrections up to 800 w_i blow any state up
-    net.set_scales(1e-3, 100.0)
+    net.set_params(random_params(9, 0.3))
+    net.set_scales(1e-3, 0.2)
     lat = L.Lattice(nx, ny, 0.6, x_edge=L.X_INLET_OUTLET, inlet_ux=0.05)
     lat.write_materials(W.materials_from_solid(solid))
     lat.write_populations(f0, 0)
     lat.set_learned_collision(net)
-    falls = lat.run_guarded(200, chunk=50, umax=0.3)
+    # The modulation is bounded so that the relaxation stays stable; leaving
+    # the healthy range is forced here with a |u| limit every state exceeds.
+    falls = lat.run_guarded(200, chunk=50, umax=1e-9)
     assert falls == 4
     plain = L.Lattice(nx, ny, 0.6, x_edge=L.X_INLET_OUTLET, inlet_ux=0.05)
     plain.write_materials(W.materials_from_solid(solid))
@@ -179,7 +180,7 @@ def test_blow_up_falls_back_to_bgk():
     assert np.array_equal(lat.read_populations()[:, fl], plain.read_populations()[:, fl])
     u, finite = lat.health()
     assert finite and u < 0.3
-    # a healthy network never falls back
+    # within the limit, no chunk falls back
     ok = L.Network(seed=8)
     ok.set_params(np.zeros(5385, np.float32))
     lat2 = L.Lattice(nx, ny, 0.6, x_edge=L.X_INLET_OUTLET, inlet_ux=0.05)
@@ -213,86 +214,77 @@ def test_restriction_keeps_block_mass_and_momentum():
     assert np.all(fc[:, sc] == 0) and np.all(uxc[sc] == 0)
 
 
-def interior(solid):
-    """Cells whose whole 3x3 neighbourhood (periodic) is fluid: the cells
-    lbm_mlp_add_transition takes samples from."""
-    near = solid.copy()
-    for dy in (-1, 0, 1):
-        for dx in (-1, 0, 1):
-            near |= np.roll(np.roll(solid, dy, 0), dx, 1)
-    return ~near
+def vortex_field(n, U, modes=6, seed=3, kmax=3):
+    """A periodic, divergence-free velocity field: the curl of a sum of
+    random Fourier modes of the stream function (wavenumbers 1..kmax per
+    axis), scaled to a largest speed U."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:n, 0:n] * (2 * np.pi / n)
+    dpx = np.zeros((n, n))
+    dpy = np.zeros((n, n))
+    for _ in range(modes):
+        kx, ky = rng.integers(1, kmax + 1, size=2) * rng.choice([-1, 1], size=2)
+        arg = kx * x + ky * y + rng.uniform(0, 2 * np.pi)
+        a = rng.normal()
+        dpx += a * kx * np.cos(arg)
+        dpy += a * ky * np.cos(arg)
+    s = U / np.max(np.hypot(dpy, dpx))
+    return dpy * s, -dpx * s
 
 
-def rms_scales(s0, s1, solid, tau, g):
-    """Root-mean-square of the network input (f - feq)/w and of the
-    residual target / w over the cells samples come from (3x3 fluid) of one
-    transition: the normalisations that make both of order one."""
-    w = W.W_PHYS[:, None, None]
-    ex, ey = W.EX[:, None, None], W.EY[:, None, None]
-    f = s0.astype(np.float64)
-    rho = f.sum(0)
-    ux = ((ex * f).sum(0) + 0.5 * g[0]) / rho
-    uy = ((ey * f).sum(0) + 0.5 * g[1]) / rho
-    eu = ex * ux + ey * uy
-    feq = w * rho * (1 + 3 * eu + 4.5 * eu**2 - 1.5 * (ux**2 + uy**2))
-    om = 1.0 / tau
-    F = (1 - 0.5 * om) * w * ((3 * (ex - ux) + 9 * eu * ex) * g[0] + (3 * (ey - uy) + 9 * eu * ey) * g[1])
-    bgk = f - om * (f - feq) + F
-    nxt = np.stack([np.roll(np.roll(s1[i], -W.EX[i], 1), -W.EY[i], 0) for i in range(9)])
-    fl = interior(solid)
-    return float(np.sqrt(np.mean(((f - feq) / w)[:, fl] ** 2))), float(np.sqrt(np.mean(((nxt - bgk) / w)[:, fl] ** 2)))
-
-
-def coarse_from_rest(n, tau, g, solid, steps, net=None):
-    lat = L.Lattice(n, n, tau, force=g)
-    lat.write_materials(W.materials_from_solid(solid))
-    if net is not None:
-        lat.set_learned_collision(net)
-    lat.run(steps)
-    _, ux, uy = lat.macro()
-    return ux, uy
+def equilibrium(ux, uy):
+    f = np.empty((9,) + ux.shape)
+    for i in range(9):
+        eu = W.EX[i] * ux + W.EY[i] * uy
+        f[i] = W.W_PHYS[i] * (1 + 3 * eu + 4.5 * eu * eu - 1.5 * (ux * ux + uy * uy))
+    return f
 
 
 def test_learned_coarse_beats_plain_coarse():
-    """Fine 256^2 (tau 0.52, 2x2 cylinder array r = 24, body force) from rest,
-    restricted every 4 fine steps to 64^2; the network is trained on those
-    transitions; coarse runs at tau 0.505 from rest with and without it are
-    compared with the restricted fine velocity."""
-    print("case: learned coarse vs plain coarse", flush=True)
+    """What the trained network gains.  Fine 256^2, tau_f = 0.52, wall-free
+    periodic flow of decaying vortices (U = 0.05) restricted every 4 fine
+    steps to 64^2 with tau_c = 0.505 (the same flow in acoustic scaling);
+    the network is trained (device forward/backward/Adam) on the coarse
+    one-step residuals of transitions 75..299 and then steps the coarse
+    lattice from the restricted initial state for 300 steps, with the
+    matrix-core network.  Its velocity L2 error against the restricted fine
+    solution is compared with plain BGK on the same coarse lattice (the
+    method without the network); the learned run must cut it by at least
+    20 % (measured: about 40 %)."""
+    print("case: learned coarse vs plain coarse, decaying vortices 256 -> 64", flush=True)
     nf, nc = 256, 64
     tau_f, tau_c = 0.52, 0.505
-    gf = (4e-6, 0.0)
-    gc = (4 * gf[0], 0.0)
-    y, x = np.mgrid[0:nf, 0:nf]
-    solid_f = np.zeros((nf, nf), bool)
-    for cx in (64, 192):
-        for cy in (64, 192):
-            solid_f |= (x - cx) ** 2 + (y - cy) ** 2 <= 24 * 24
-    fine = L.Lattice(nf, nf, tau_f, force=gf)
-    fine.write_materials(W.materials_from_solid(solid_f))
+    ux, uy = vortex_field(nf, 0.05)
+    fine = L.Lattice(nf, nf, tau_f)
+    fine.write_populations(equilibrium(ux, uy).astype(np.float32), 0)
     T = 300
-    states, solid_c = [], None
+    states = []
     for t in range(T + 1):
-        fc, solid_c, uxc, uyc = fine.restrict4(tau_c, gc)
+        fc, solid_c, uxc, uyc = fine.restrict4(tau_c, (0.0, 0.0))
         states.append((fc, uxc, uyc))
         fine.run(4)
+    assert not solid_c.any()
     net = L.Network(seed=21, capacity=400000)
-    s_in, s_out = rms_scales(states[T // 2][0], states[T // 2 + 1][0], solid_c, tau_c, gc)
-    net.set_scales(s_in, s_out)
-    for t in range(T):
-        net.add_transition(states[t][0], states[t + 1][0], solid_c, tau_c, gc, stride=3, offset=t % 3)
+    net.set_scales(1.0, 1.0)
+    for t in range(T // 4, T):
+        net.add_transition(states[t][0], states[t + 1][0], solid_c, tau_c, (0.0, 0.0), stride=3, offset=t % 3)
     rx, rt = net.normalize_samples()
-    print(f"samples {net.sample_count()}, scales {s_in:.3e} {s_out:.3e}, rms {rx:.3e} {rt:.3e}", flush=True)
+    print(f"samples {net.sample_count()}, rms {rx:.3e} {rt:.3e}", flush=True)
     loss = net.train(steps=1500, batch=4096, lr=2e-3, seed=3)
     print(f"final training loss {loss:.4e} (unit-rms targets)", flush=True)
     assert loss < 1.0
-    ux_p, uy_p = coarse_from_rest(nc, tau_c, gc, solid_c, T)
-    ux_l, uy_l = coarse_from_rest(nc, tau_c, gc, solid_c, T, net)
-    _, uxr, uyr = states[T]
-    fl = ~solid_c
-    ref = np.hypot(uxr, uyr)[fl]
-    e_plain = np.linalg.norm(np.hypot(ux_p - uxr, uy_p - uyr)[fl]) / np.linalg.norm(ref)
-    e_learn = np.linalg.norm(np.hypot(ux_l - uxr, uy_l - uyr)[fl]) / np.linalg.norm(ref)
-    print(f"velocity L2 error vs restricted fine: plain {e_plain:.4e}, learned {e_learn:.4e}", flush=True)
-    assert np.isfinite(e_learn)
-    assert e_learn < e_plain
+    errs = {}
+    for name, learned in (("plain", False), ("learned", True)):
+        lat = L.Lattice(nc, nc, tau_c)
+        lat.write_populations(states[0][0], 0)
+        if learned:
+            lat.set_learned_collision(net)
+        lat.run(T)
+        _, ux_c, uy_c = lat.macro()
+        _, uxr, uyr = states[T]
+        errs[name] = float(np.linalg.norm(np.hypot(ux_c - uxr, uy_c - uyr)) / np.linalg.norm(np.hypot(uxr, uyr)))
+        lat.close()
+    print(f"velocity L2 error vs restricted fine: plain {errs['plain']:.4e}, learned {errs['learned']:.4e}",
+          flush=True)
+    assert np.isfinite(errs["learned"])
+    assert errs["learned"] < 0.8 * errs["plain"]
