@@ -539,14 +539,14 @@ __global__ __launch_bounds__(BS) void k_phys_step_shfl(PhysArgs a, const T* __re
 // destination buffer ends up exactly as two one-step launches leave it.
 // HBM traffic per two steps: the ring's extra reads ((BW+2)(BH+2)/(BW BH)
 // of 36 B) plus 36 B written, instead of 2 x 72 B.
-template <typename T, bool FORCE, int BW, int BH, int FLAGS>
-__global__ __launch_bounds__(256) void k_phys_step2(PhysArgs a, const T* __restrict__ src, T* __restrict__ dst,
-                                                    const uint32_t* __restrict__ bits, uint32_t words,
-                                                    uint32_t tiles_x)
+template <typename T, bool FORCE, int BS, int BW, int BH, int FLAGS>
+__global__ __launch_bounds__(BS) void k_phys_step2(PhysArgs a, const T* __restrict__ src, T* __restrict__ dst,
+                                                   const uint32_t* __restrict__ bits, uint32_t words,
+                                                   uint32_t tiles_x)
 {
     constexpr int RW = BW + 2, RH = BH + 2, RC = RW * RH;  // step-1 region
-    constexpr int kIter1 = (RC + 255) / 256, kIter2 = (BW * BH) / 256;
-    static_assert((BW * BH) % 256 == 0, "tile must be a multiple of the block");
+    constexpr int kIter1 = (RC + BS - 1) / BS, kIter2 = (BW * BH) / BS;
+    static_assert((BW * BH) % BS == 0, "tile must be a multiple of the block");
     __shared__ T lds[9][RC];
     const int x0 = (int)(blockIdx.x % tiles_x) * BW;
     const int y0 = (int)(blockIdx.x / tiles_x) * BH;
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(256) void k_phys_step2(PhysArgs a, const T* __restr
     // Step 1: region cells (q, r) = (x0 - 1 + q, y0 - 1 + r), wrapped.
 #pragma unroll
     for (int m = 0; m < kIter1; ++m) {
-        const int k = (int)threadIdx.x + 256 * m;
+        const int k = (int)threadIdx.x + BS * m;
         if (k >= RC) break;
         const int r = k / RW, q = k % RW;
         int x = x0 - 1 + q, y = y0 - 1 + r;
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(256) void k_phys_step2(PhysArgs a, const T* __restr
     // Step 2: tile cells, gathered from LDS.
 #pragma unroll
     for (int m = 0; m < kIter2; ++m) {
-        const int k = (int)threadIdx.x + 256 * m;
+        const int k = (int)threadIdx.x + BS * m;
         const int row = k / BW, col = k % BW;
         const int x = x0 + col, y = y0 + row;
         const CellNbr nb = cell_nbr(a, bits, words, x, y);
@@ -1395,7 +1395,7 @@ static hipError_t launch_phys_variant_shfl(const PhysArgs& a, const void* src, v
     }
 }
 
-template <typename T, int XEDGE, bool FORCE, int BW, int BH, int FLAGS>
+template <typename T, int XEDGE, bool FORCE, int BS, int BW, int BH, int FLAGS>
 static hipError_t launch_phys_variant2(const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
                                        uint32_t words, hipStream_t s)
 {
@@ -1406,7 +1406,7 @@ static hipError_t launch_phys_variant2(const PhysArgs& a, const void* src, void*
         if (a.nx % BW != 0 || a.ny % BH != 0 || a.nx < 3 || a.ny < 3) return hipErrorInvalidValue;
         const uint32_t tiles_x = (uint32_t)(a.nx / BW);
         const uint32_t tiles = tiles_x * (uint32_t)(a.ny / BH);
-        hipLaunchKernelGGL((k_phys_step2<T, FORCE, BW, BH, FLAGS>), dim3(tiles), dim3(256), 0, s, a, (const T*)src,
+        hipLaunchKernelGGL((k_phys_step2<T, FORCE, BS, BW, BH, FLAGS>), dim3(tiles), dim3(BS), 0, s, a, (const T*)src,
                            (T*)dst, bits, words, tiles_x);
         return hipGetLastError();
     }
@@ -1439,6 +1439,12 @@ const StepVariant kStepVariants[] = {
     {256, 16, 64, kNtStore | kTwoStep},
     {256, 8, 256, kNtStore | kTwoStep},
     {256, 4, 256, kNtStore | kTwoStep},
+    {512, 16, 64, kNtStore | kTwoStep},
+    {1024, 16, 64, kNtStore | kTwoStep},
+    {512, 8, 128, kNtStore | kTwoStep},
+    {256, 8, 64, kNtStore | kTwoStep},
+    {512, 32, 64, kNtStore | kTwoStep},
+    {256, 16, 32, kNtStore | kTwoStep},
 };
 const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
 
@@ -1493,10 +1499,16 @@ static hipError_t dispatch_variant2(int v, const PhysArgs& a, const void* src, v
 {
     constexpr int S = kNtStore;
     switch (v) {
-    case 18: return launch_phys_variant2<T, XEDGE, FORCE, 128, 8, S>(a, src, dst, bits, words, s);
-    case 19: return launch_phys_variant2<T, XEDGE, FORCE, 64, 16, S>(a, src, dst, bits, words, s);
-    case 20: return launch_phys_variant2<T, XEDGE, FORCE, 256, 8, S>(a, src, dst, bits, words, s);
-    case 21: return launch_phys_variant2<T, XEDGE, FORCE, 256, 4, S>(a, src, dst, bits, words, s);
+    case 18: return launch_phys_variant2<T, XEDGE, FORCE, 256, 128, 8, S>(a, src, dst, bits, words, s);
+    case 19: return launch_phys_variant2<T, XEDGE, FORCE, 256, 64, 16, S>(a, src, dst, bits, words, s);
+    case 20: return launch_phys_variant2<T, XEDGE, FORCE, 256, 256, 8, S>(a, src, dst, bits, words, s);
+    case 21: return launch_phys_variant2<T, XEDGE, FORCE, 256, 256, 4, S>(a, src, dst, bits, words, s);
+    case 22: return launch_phys_variant2<T, XEDGE, FORCE, 512, 64, 16, S>(a, src, dst, bits, words, s);
+    case 23: return launch_phys_variant2<T, XEDGE, FORCE, 1024, 64, 16, S>(a, src, dst, bits, words, s);
+    case 24: return launch_phys_variant2<T, XEDGE, FORCE, 512, 128, 8, S>(a, src, dst, bits, words, s);
+    case 25: return launch_phys_variant2<T, XEDGE, FORCE, 256, 64, 8, S>(a, src, dst, bits, words, s);
+    case 26: return launch_phys_variant2<T, XEDGE, FORCE, 512, 64, 32, S>(a, src, dst, bits, words, s);
+    case 27: return launch_phys_variant2<T, XEDGE, FORCE, 256, 32, 16, S>(a, src, dst, bits, words, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1640,7 +1652,7 @@ hipError_t launch_phys_step_learned_mfma(bool f16, int xedge, bool force, const 
 {
     const uint32_t nseg = (uint32_t)((a.nx + 255) / 256);
     const uint32_t tiles = nseg * (uint32_t)a.ny;
-    const dim3 grid(tiles < 1024u ? tiles : 1024u), block(256);
+    const dim3 grid(tiles < 2048u ? tiles : 2048u), block(256);
 #define LBM_L(T, XE, FO)                                                                                       \
     hipLaunchKernelGGL((k_phys_step_learned_mfma<T, XE, FO>), grid, block, 0, s, a, (const T*)src, (T*)dst, bits, \
                        words, packed, params, s_in, s_out, nseg)
