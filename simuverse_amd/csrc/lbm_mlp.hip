@@ -104,13 +104,17 @@ __global__ __launch_bounds__(256) void k_mlp_forward_bf16(const __bf16* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Loss gradient: a block of 256 threads takes 64 samples; thread (s, q)
-// = (tid % 64, tid / 64) computes hidden units 16q..16q+15 of sample s.
-// Activations are staged in LDS (rows padded to 65 floats), then the block
-// forms its weight-gradient partial as sample-sums over LDS.
+// Loss gradient: a block of 256 threads takes chunks of 64 samples (chunks
+// b, b + grid, ...); for a chunk, thread (s, q) = (tid % 64, tid / 64)
+// computes hidden units 16q..16q+15 of sample s, activations are staged in
+// LDS (rows padded to 65 floats), and each thread adds the chunk's
+// sample-sums of its parameters p = tid + 256 r to registers.  One partial
+// per block, summed over the (at most kGradBlocks) blocks in block order.
 // ---------------------------------------------------------------------------
-constexpr int kTS = 64;       // samples per block
+constexpr int kTS = 64;       // samples per chunk
 constexpr int kPadH = kMlpH + 1;
+constexpr int kGradBlocks = 1024;
+constexpr int kPerThread = (kMlpParams + 255) / 256;
 
 __global__ __launch_bounds__(256) void k_mlp_grad_partial(const float* __restrict__ params,
                                                           const float* __restrict__ X, const float* __restrict__ T,
@@ -125,7 +129,14 @@ __global__ __launch_bounds__(256) void k_mlp_grad_partial(const float* __restric
     const int tid = threadIdx.x;
     for (int k = tid; k < kMlpParams; k += 256) P[k] = params[k];
     const int s = tid & (kTS - 1), q = tid >> 6;
-    const int g = blockIdx.x * kTS + s;  // position in the batch
+    float acc[kPerThread];
+#pragma unroll
+    for (int r = 0; r < kPerThread; ++r) acc[r] = 0.0f;
+    double lblock = 0.0;
+    const int nchunks = (batch + kTS - 1) / kTS;
+    for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    __syncthreads();  // the previous chunk's LDS reads are done
+    const int g = chunk * kTS + s;  // position in the batch
     const bool live = g < batch;
     // sample index: the batch position itself, or a hashed draw from n
     const int idx = !live ? 0 : (use_index ? (int)(mix64(seed + 0x9E3779B97F4A7C15ull * (uint64_t)(g + 1)) % (uint64_t)n) : g);
@@ -183,36 +194,43 @@ __global__ __launch_bounds__(256) void k_mlp_grad_partial(const float* __restric
         D1[s][j] = d * gelu_grad(A1[s][j]);
     }
     __syncthreads();
-    // weight-gradient partials of this block: sums over its samples
-    float* out = partial + (size_t)blockIdx.x * kMlpParams;
-    for (int p = tid; p < kMlpParams; p += 256) {
-        float acc = 0.0f;
+    // this chunk's sample-sums of the weight gradients, into registers
+#pragma unroll
+    for (int r = 0; r < kPerThread; ++r) {
+        const int p = tid + 256 * r;
+        if (p >= kMlpParams) break;
+        float c = 0.0f;
         if (p < kOffB1) {
             const int j = p / kMlpIn, i = p % kMlpIn;
-            for (int t = 0; t < kTS; ++t) acc += D1[t][j] * xs[t][i];
+            for (int t = 0; t < kTS; ++t) c += D1[t][j] * xs[t][i];
         } else if (p < kOffW2) {
             const int j = p - kOffB1;
-            for (int t = 0; t < kTS; ++t) acc += D1[t][j];
+            for (int t = 0; t < kTS; ++t) c += D1[t][j];
         } else if (p < kOffB2) {
             const int j = (p - kOffW2) / kMlpH, k = (p - kOffW2) % kMlpH;
-            for (int t = 0; t < kTS; ++t) acc += D2[t][j] * H1[t][k];
+            for (int t = 0; t < kTS; ++t) c += D2[t][j] * H1[t][k];
         } else if (p < kOffW3) {
             const int j = p - kOffB2;
-            for (int t = 0; t < kTS; ++t) acc += D2[t][j];
+            for (int t = 0; t < kTS; ++t) c += D2[t][j];
         } else if (p < kOffB3) {
             const int m = (p - kOffW3) / kMlpH, k = (p - kOffW3) % kMlpH;
-            for (int t = 0; t < kTS; ++t) acc += dy[t][m] * H2[t][k];
+            for (int t = 0; t < kTS; ++t) c += dy[t][m] * H2[t][k];
         } else {
             const int m = p - kOffB3;
-            for (int t = 0; t < kTS; ++t) acc += dy[t][m];
+            for (int t = 0; t < kTS; ++t) c += dy[t][m];
         }
-        out[p] = acc;
+        acc[r] += c;
     }
-    if (tid == 0) {
-        double l = 0.0;
-        for (int t = 0; t < kTS; ++t) l += lsum[t];
-        loss_part[blockIdx.x] = l;
+    if (tid == 0)
+        for (int t = 0; t < kTS; ++t) lblock += lsum[t];
+    }  // chunks
+    float* out = partial + (size_t)blockIdx.x * kMlpParams;
+#pragma unroll
+    for (int r = 0; r < kPerThread; ++r) {
+        const int p = tid + 256 * r;
+        if (p < kMlpParams) out[p] = acc[r];
     }
+    if (tid == 0) loss_part[blockIdx.x] = lblock;
 }
 
 // grad[p] = sum over blocks of partial[b][p] in block order.
@@ -386,7 +404,7 @@ void mlp_free(lbm_mlp* m)
 // samples, or hashed draws when use_index) into m->grad; loss into m->loss.
 int mlp_grad(lbm_mlp* m, const float* X, const float* T, int n, int batch, uint64_t seed, int use_index)
 {
-    const int nb = (batch + kTS - 1) / kTS;
+    const int nb = std::min((batch + kTS - 1) / kTS, kGradBlocks);
     if (nb > m->partial_blocks) {
         if (m->partial) (void)hipFree(m->partial);
         if (m->loss_part) (void)hipFree(m->loss_part);
