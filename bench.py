@@ -44,8 +44,9 @@ def log(*a):
 def traffic_from_profiles(workload: str, variant: int):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, scaled by the copy-kernel calibration), as committed in
-    profiles/<workload>_pmc.md for the same step-kernel variant, or None."""
-    p = ROOT / "profiles" / f"{workload}_pmc.md"
+    profiles/<workload>_v<variant>_pmc.md for the step-kernel variant used, or
+    None."""
+    p = ROOT / "profiles" / f"{workload}_v{variant}_pmc.md"
     if not p.exists():
         return None
     text = p.read_text()
@@ -351,7 +352,7 @@ def main():
     if args.variant >= 0:
         lat.set_variant(args.variant)
     elif not args.no_autotune:
-        v, ms = lat.autotune(steps=20)
+        v, ms = lat.autotune(steps=200)
         log(f"autotune: variant {v} {L.variant_shapes()[v]} {ms*1e3:.1f} us/step")
     variant = lat.variant()
 
