@@ -11,7 +11,10 @@ allows:
   binary32: deviations ~1e-3 carry ~1e-10 rounding, the body force adds
             1e-7 per step, so u (~1e-4 .. 1e-3) keeps ~1e-4 relative;
   binary16: 11 significant bits on deviations (2^-11 ~ 5e-4 relative of
-            each), accumulating over the steps to a few percent in u.
+            each), accumulating over the steps to ~1e-3 in u.
+Measured on MI355X: binary32 u rel L2 7.6e-5 / 2.7e-4 and binary16
+4.5e-4 / 1.3e-3 at 1000 / 10 000 steps (rho 1.3e-7 / 1.8e-7 and
+4.3e-7 / 4.1e-6).
 """
 import json
 
@@ -64,7 +67,7 @@ def test_f16_and_f32_storage_vs_fp64_at_1000_and_10000_steps():
         rows.append(row)
         print("precision", json.dumps(row), flush=True)
     for row in rows:
-        assert row["f32"]["rho_l2"] < 1e-6 and row["f32"]["u_l2"] < 5e-4
-        assert row["f16dev"]["rho_l2"] < 1e-4 and row["f16dev"]["u_l2"] < 1e-1
+        assert row["f32"]["rho_l2"] < 1e-6 and row["f32"]["u_l2"] < 1e-3
+        assert row["f16dev"]["rho_l2"] < 1e-4 and row["f16dev"]["u_l2"] < 1e-2
         # binary16 storage is the coarser of the two
         assert row["f16dev"]["u_l2"] > row["f32"]["u_l2"]
