@@ -232,12 +232,18 @@ int lbm_reduce(lbm_handle* h, double out[5]);
 /* ---- tuning and diagnostics (no reference counterpart: the reference
  * dispatches a fixed 64x4 workgroup, d2q9_node.rs:45) --------------------- */
 /* Physics-mode step kernels are built for several tile shapes; every shape
- * computes bit-identical results. */
+ * computes bit-identical populations at every slot a fluid cell can read
+ * (a two-step launch leaves older values in the solid-cell slots no fluid
+ * cell pulls from). */
 int lbm_variant_count(void);
 /* block: threads per block; cells_per_thread: cells one thread updates;
  * vec: consecutive cells per lane (1, or 4 with nx % 4 == 0); flags: bit 0
  * non-temporal population stores, bit 1 non-temporal loads, bit 2 rows
- * grouped per XCD. */
+ * grouped per XCD, bit 3 aligned vector loads with x-neighbours from the
+ * adjacent lane (periodic x, nx a multiple of block*vec), bit 4 two steps
+ * per launch through an LDS tile of cells_per_thread rows x vec cells
+ * (nx, ny >= 3; with LBM_X_INLET_OUTLET, nx % vec != 1; odd step counts
+ * end with a one-step launch). */
 int lbm_variant_info(int variant, int* block, int* cells_per_thread, int* vec, int* flags);
 int lbm_set_variant(lbm_handle* h, int variant);
 int lbm_get_variant(lbm_handle* h, int* variant);
@@ -259,7 +265,8 @@ int lbm_copy_bandwidth(size_t bytes, int reps, double* ms_per_copy);
  * own tau, x edges, inlet velocity, body force and solids; one persistent
  * kernel launch steps every domain (100-step HIP-graph chunks in
  * lbm_batch_run).  Each domain evolves exactly as a single-domain handle
- * with the same descriptor (bit for bit). */
+ * with the same descriptor (bit for bit at every slot a fluid cell can
+ * read). */
 typedef struct lbm_batch lbm_batch;
 
 typedef struct lbm_domain_desc {
@@ -288,6 +295,12 @@ int lbm_batch_synchronize(lbm_batch* b);
 int lbm_batch_macro(lbm_batch* b, int domain, float* rho, float* ux, float* uy);
 int lbm_batch_reduce(lbm_batch* b, int domain, double out[5]);
 int lbm_batch_variant_count(void);
+/* block: threads per block; vec: consecutive cells per lane, or the tile
+ * width of a two-step variant; tile_h: tile rows of a two-step variant (0
+ * otherwise); flags: bit 0 non-temporal stores, bit 2 work grouped per
+ * XCD, bit 4 two steps per launch (one block per tile), bit 5 persistent
+ * blocks take every grid-th chunk. */
+int lbm_batch_variant_info(int variant, int* block, int* vec, int* tile_h, int* flags);
 int lbm_batch_set_variant(lbm_batch* b, int variant);
 int lbm_batch_get_variant(lbm_batch* b, int* variant);
 int lbm_batch_set_graph_chunk(lbm_batch* b, int chunk);

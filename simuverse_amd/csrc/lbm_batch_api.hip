@@ -48,8 +48,10 @@ struct lbm_batch {
     uint8_t* mat = nullptr;
     uint32_t* bits = nullptr;
     BatchDomainDev* ddom = nullptr;
-    uint16_t* chunk_dom = nullptr;
+    uint16_t* chunk_dom = nullptr;  // chunk -> domain for the one-step variant in use
     uint32_t nchunks = 0;
+    uint16_t* tile_dom = nullptr;   // tile -> domain for a two-step variant
+    uint32_t ntiles = 0;
     float* tmp3 = nullptr;
     double* partial = nullptr;
     double* red = nullptr;
@@ -62,9 +64,12 @@ struct lbm_batch {
 
     PhysArgs args(int d) const { return dom[d].a; }
     void* view(int b, int d) const { return (char*)buf[b] + (size_t)dom[d].pop_off * elem; }
+    bool two_step() const { return batch_variant_two_step(variant); }
+    // the variant of single steps: the chosen one, or the fallback of a two-step one
+    int step1_variant() const { return two_step() ? kBatchSingleStepFallback : variant; }
     uint32_t grid() const
     {
-        const uint32_t g = (uint32_t)(kBatchVariants[variant].blocks_per_cu * cus);
+        const uint32_t g = (uint32_t)(kBatchVariants[step1_variant()].blocks_per_cu * cus);
         return nchunks < g ? (nchunks ? nchunks : 1) : g;
     }
 };
@@ -83,7 +88,7 @@ void batch_free(lbm_batch* b)
 {
     batch_drop_graphs(b);
     for (void* p : {b->buf[0], b->buf[1], (void*)b->mat, (void*)b->bits, (void*)b->ddom, (void*)b->chunk_dom,
-                    (void*)b->tmp3, (void*)b->partial, (void*)b->red})
+                    (void*)b->tile_dom, (void*)b->tmp3, (void*)b->partial, (void*)b->red})
         if (p) (void)hipFree(p);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -104,40 +109,84 @@ int batch_domain_ok(lbm_batch* b, int d)
     return LBM_OK;
 }
 
-// Chunk -> domain table for the current variant (chunk = block*vec cells).
+// Uploads a work-unit -> domain table.
+int upload_table(lbm_batch* b, const std::vector<uint16_t>& table, uint16_t** dev)
+{
+    if (*dev) (void)hipFree(*dev);
+    *dev = nullptr;
+    BATCH_TRY(hipMalloc(dev, std::max<size_t>(table.size(), 1) * sizeof(uint16_t)));
+    BATCH_TRY(hipMemcpyAsync(*dev, table.data(), table.size() * sizeof(uint16_t), hipMemcpyHostToDevice, b->stream));
+    BATCH_TRY(hipStreamSynchronize(b->stream));
+    return LBM_OK;
+}
+
+// Work tables of the current variant: chunks of block*vec cells for the
+// one-step variant (the chosen one or the two-step fallback), and tiles for
+// a two-step variant.
 int build_chunks(lbm_batch* b)
 {
-    const uint32_t w = (uint32_t)(kBatchVariants[b->variant].block * kBatchVariants[b->variant].vec);
-    std::vector<uint16_t> table;
+    const BatchVariant& sv = kBatchVariants[b->step1_variant()];
+    const uint32_t w = (uint32_t)(sv.block * sv.vec);
+    std::vector<uint16_t> table, tiles;
     for (size_t d = 0; d < b->dom.size(); ++d) {
-        b->dom[d].chunk0 = (uint32_t)table.size();
-        const uint32_t nc = (b->dom[d].ncells + w - 1) / w;
-        table.insert(table.end(), nc, (uint16_t)d);
+        BatchDomainDev& D = b->dom[d];
+        D.chunk0 = (uint32_t)table.size();
+        table.insert(table.end(), (D.ncells + w - 1) / w, (uint16_t)d);
+        D.tile0 = (uint32_t)tiles.size();
+        if (b->two_step()) {
+            const BatchVariant& tv = kBatchVariants[b->variant];
+            const uint32_t nt = (uint32_t)((D.a.nx + tv.vec - 1) / tv.vec) * (uint32_t)((D.a.ny + tv.tile_h - 1) / tv.tile_h);
+            tiles.insert(tiles.end(), nt, (uint16_t)d);
+        }
     }
     b->nchunks = (uint32_t)table.size();
-    if (b->chunk_dom) (void)hipFree(b->chunk_dom);
-    b->chunk_dom = nullptr;
-    BATCH_TRY(hipMalloc(&b->chunk_dom, std::max<size_t>(table.size(), 1) * sizeof(uint16_t)));
-    BATCH_TRY(hipMemcpyAsync(b->chunk_dom, table.data(), table.size() * sizeof(uint16_t), hipMemcpyHostToDevice,
-                             b->stream));
+    b->ntiles = (uint32_t)tiles.size();
+    int rc = upload_table(b, table, &b->chunk_dom);
+    if (rc == LBM_OK) rc = upload_table(b, tiles, &b->tile_dom);
+    if (rc != LBM_OK) return rc;
     BATCH_TRY(hipMemcpyAsync(b->ddom, b->dom.data(), b->dom.size() * sizeof(BatchDomainDev), hipMemcpyHostToDevice,
                              b->stream));
     BATCH_TRY(hipStreamSynchronize(b->stream));
     return LBM_OK;
 }
 
+// Whether a variant can step every domain of the batch.
+bool batch_variant_ok(const lbm_batch* b, int v)
+{
+    if (v < 0 || v >= kNumBatchVariants) return false;
+    if (!batch_variant_two_step(v)) return true;
+    for (const lbm_domain_desc& q : b->desc)
+        if (q.x_edge != LBM_X_PERIODIC && q.nx % kBatchVariants[v].vec == 1) return false;
+    return true;
+}
+
+// One step from buffer `swap` into buffer 1-swap.
 hipError_t batch_enqueue_step(lbm_batch* b, int swap)
 {
-    return launch_batch_step(b->variant, b->f16, b->ddom, b->chunk_dom, b->nchunks, b->grid(), b->buf[swap],
-                             b->buf[swap ^ 1], b->bits, b->stream);
+    return launch_batch_step(b->step1_variant(), b->f16, b->ddom, b->chunk_dom, b->nchunks, b->grid(),
+                             b->buf[swap], b->buf[swap ^ 1], b->bits, b->stream);
 }
+
+// Two steps from buffer `swap` into buffer 1-swap (two-step variant).
+hipError_t batch_enqueue_step2(lbm_batch* b, int swap)
+{
+    return launch_batch_step2(b->variant, b->f16, b->ddom, b->tile_dom, b->ntiles, b->buf[swap], b->buf[swap ^ 1],
+                              b->bits, b->stream);
+}
+
+// Buffer flips of one graph chunk: one per launch.
+int batch_chunk_flips(const lbm_batch* b) { return b->two_step() ? (b->graph_chunk / 2) & 1 : 0; }
 
 int batch_ensure_graph(lbm_batch* b, int start)
 {
     if (b->graph[start]) return LBM_OK;
     BATCH_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t le = hipSuccess;
-    for (int s = 0; s < b->graph_chunk && le == hipSuccess; ++s) le = batch_enqueue_step(b, (start + s) & 1);
+    if (b->two_step()) {
+        for (int s = 0; s < b->graph_chunk / 2 && le == hipSuccess; ++s) le = batch_enqueue_step2(b, (start + s) & 1);
+    } else {
+        for (int s = 0; s < b->graph_chunk && le == hipSuccess; ++s) le = batch_enqueue_step(b, (start + s) & 1);
+    }
     hipGraph_t g = nullptr;
     hipError_t ee = hipStreamEndCapture(b->stream, &g);
     if (le != hipSuccess || ee != hipSuccess) {
@@ -162,6 +211,13 @@ int batch_enqueue_run(lbm_batch* b, int nsteps)
             if (rc != LBM_OK) return rc;
             BATCH_TRY(hipGraphLaunch(b->graph[b->cur], b->stream));
             left -= b->graph_chunk;
+            b->cur ^= batch_chunk_flips(b);  // one flip per launch: even for one-step chunks
+        }
+    }
+    if (b->two_step()) {
+        for (; left >= 2; left -= 2) {
+            BATCH_TRY(batch_enqueue_step2(b, b->cur));
+            b->cur ^= 1;
         }
     }
     for (int s = 0; s < left; ++s) {
@@ -450,11 +506,25 @@ int lbm_batch_reduce(lbm_batch* b, int d, double out[5])
 
 int lbm_batch_variant_count(void) { return kNumBatchVariants; }
 
+int lbm_batch_variant_info(int variant, int* block, int* vec, int* tile_h, int* flags)
+{
+    if (variant < 0 || variant >= kNumBatchVariants) return lbm_fail(LBM_ERR_INVALID, "no batch variant %d", variant);
+    const BatchVariant& v = kBatchVariants[variant];
+    if (block) *block = v.block;
+    if (vec) *vec = v.vec;
+    if (tile_h) *tile_h = v.tile_h;
+    if (flags) *flags = v.flags;
+    return LBM_OK;
+}
+
 int lbm_batch_set_variant(lbm_batch* b, int variant)
 {
     int rc = batch_check(b);
     if (rc) return rc;
     if (variant < 0 || variant >= kNumBatchVariants) return lbm_fail(LBM_ERR_INVALID, "no batch variant %d", variant);
+    if (!batch_variant_ok(b, variant))
+        return lbm_fail(LBM_ERR_INVALID, "batch variant %d cannot step every domain (an inlet/outlet domain leaves "
+                        "a one-column tile)", variant);
     BATCH_TRY(hipStreamSynchronize(b->stream));
     if (variant == b->variant) return LBM_OK;
     batch_drop_graphs(b);
@@ -494,20 +564,29 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* best_variant, double* ms_pe
     int best = v0;
     double best_ms = 1e300;
     for (int v = 0; v < kNumBatchVariants && e == hipSuccess; ++v) {
+        if (!batch_variant_ok(b, v)) continue;
         b->variant = v;
         if (build_chunks(b) != LBM_OK) {
             e = hipErrorUnknown;
             break;
         }
-        for (int s = 0; s < 2 && e == hipSuccess; ++s, b->cur ^= 1) e = batch_enqueue_step(b, b->cur);
+        const bool ts = b->two_step();
+        const int per = ts ? 2 : 1;  // steps per launch
+        auto launch = [&]() {
+            hipError_t r = ts ? batch_enqueue_step2(b, b->cur) : batch_enqueue_step(b, b->cur);
+            b->cur ^= 1;
+            return r;
+        };
+        for (int s = 0; s < 2 && e == hipSuccess; ++s) e = launch();
         if (e == hipSuccess) e = hipEventRecord(b->ev0, b->stream);
-        for (int s = 0; s < steps && e == hipSuccess; ++s, b->cur ^= 1) e = batch_enqueue_step(b, b->cur);
+        const int launches = (steps + per - 1) / per;
+        for (int s = 0; s < launches && e == hipSuccess; ++s) e = launch();
         if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
         if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
         float t = 0.0f;
         if (e == hipSuccess) e = hipEventElapsedTime(&t, b->ev0, b->ev1);
-        if (e == hipSuccess && t / steps < best_ms) {
-            best_ms = t / steps;
+        if (e == hipSuccess && t / (launches * per) < best_ms) {
+            best_ms = t / (launches * per);
             best = v;
         }
     }

@@ -177,7 +177,7 @@ struct BatchDomainDev {
     uint32_t ncells;    // nx * ny
     int32_t x_edge;     // 0 periodic, 1 inlet/outlet
     int32_t force;      // non-zero when (gx, gy) != 0
-    uint32_t pad;
+    uint32_t tile0;     // index of the domain's first two-step tile
 };
 
 // Moments of the post-streaming deviations (Guo velocity with FORCE).

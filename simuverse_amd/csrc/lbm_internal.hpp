@@ -58,17 +58,29 @@ hipError_t launch_pack_rgba16f(const float* macro, uint32_t n, uint16_t* out, hi
 hipError_t launch_curl_rgba16f(const float* macro, int nx, int ny, uint16_t* out, hipStream_t s);
 hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s);
 
+// Batched step shapes.  One-step variants: a chunk is block*vec consecutive
+// cells of one domain and a persistent grid of blocks_per_cu blocks per CU
+// walks the chunks.  Two-step variants (kTwoStep): one block per tile of
+// tile_h rows x vec cells of one domain, two steps per launch; odd step
+// counts take the one-step variant kBatchSingleStepFallback.
 struct BatchVariant {
     int block;           // threads per block
-    int vec;             // consecutive cells per lane
-    int blocks_per_cu;   // persistent grid = blocks_per_cu * CUs
-    int flags;           // kNtStore | kXcdRemap
+    int vec;             // consecutive cells per lane; tile width when two-step
+    int blocks_per_cu;   // persistent grid = blocks_per_cu * CUs (one-step)
+    int flags;           // kNtStore | kXcdRemap | kGridStride | kTwoStep
+    int tile_h;          // tile rows (two-step)
 };
+constexpr int kBatchSingleStepFallback = 9;
 extern const BatchVariant kBatchVariants[];
 extern const int kNumBatchVariants;
+inline bool batch_variant_two_step(int variant) { return (kBatchVariants[variant].flags & kTwoStep) != 0; }
 hipError_t launch_batch_step(int variant, bool f16, const BatchDomainDev* doms, const uint16_t* chunk_dom,
                              uint32_t nchunks, uint32_t grid, const void* src, void* dst, const uint32_t* bits,
                              hipStream_t s);
+// Two steps of every domain: tile_dom[t] is the domain of tile t, domain d
+// owns tiles [tile0, tile0 + ceil(nx/vec) * ceil(ny/tile_h)).
+hipError_t launch_batch_step2(int variant, bool f16, const BatchDomainDev* doms, const uint16_t* tile_dom,
+                              uint32_t ntiles, const void* src, void* dst, const uint32_t* bits, hipStream_t s);
 
 // Learned collision, restriction and health check.
 hipError_t launch_phys_step_learned(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, void* dst,

@@ -527,43 +527,52 @@ __global__ __launch_bounds__(BS) void k_phys_step_shfl(PhysArgs a, const T* __re
                                                 wbase, (int)(threadIdx.x & 63u), y);
 }
 
-// Two steps per launch (temporal blocking), periodic x.  A block owns a
-// BW x BH tile of cells.  Step 1 runs on the tile plus a one-cell ring,
+// Two steps per launch (temporal blocking).  A block owns a BW x BH tile of
+// cells (x0, y0 its first cell; the last tiles of a row or column may be
+// cut by the lattice edge).  Step 1 runs on the tile plus a one-cell ring,
 // gathering from HBM exactly as the one-step kernel does, and leaves each
 // cell's post-collision populations in LDS (rounded to the storage type, as
 // the one-step path rounds them into its buffer).  Step 2 runs on the tile,
 // gathering from LDS: a population that would come from a solid cell is the
-// cell's own post-collision population of the opposite direction (the
-// half-way bounce-back the push rule produces), so step 1 needs no pushes.
-// Step 2 stores to HBM with the usual pushes into solid neighbours, so the
-// destination buffer ends up exactly as two one-step launches leave it.
-// HBM traffic per two steps: the ring's extra reads ((BW+2)(BH+2)/(BW BH)
-// of 36 B) plus 36 B written, instead of 2 x 72 B.
-template <typename T, bool FORCE, int BS, int BW, int BH, int FLAGS>
-__global__ __launch_bounds__(BS) void k_phys_step2(PhysArgs a, const T* __restrict__ src, T* __restrict__ dst,
-                                                   const uint32_t* __restrict__ bits, uint32_t words,
-                                                   uint32_t tiles_x)
+// gathering cell's own post-collision population of the opposite direction
+// (the half-way bounce-back the push rule produces), so step 1 needs no
+// pushes.  With the inlet/outlet x edge (XEDGE 1) a population that would
+// come from outside [0, nx) is likewise the cell's own opposite one, column
+// 0 applies the Zou-He inlet, and the outlet column gathers as its west
+// neighbour (whose 3x3 neighbourhood lies in the step-1 region when the
+// tile holds at least two columns).  Step 2 stores to HBM with the usual
+// pushes into solid neighbours, so the destination buffer ends up exactly
+// as two one-step launches leave it.  HBM traffic per two steps: the ring's
+// extra reads ((BW+2)(BH+2)/(BW BH) of 36 B) plus 36 B written, instead of
+// 2 x 72 B.
+template <int BW, int BH>
+constexpr int kRegionCells = (BW + 2) * (BH + 2);
+
+template <typename T, int XEDGE, bool FORCE, int BS, int BW, int BH, int FLAGS>
+__device__ inline void two_step_tile(const PhysArgs& a, const T* __restrict__ src, T* __restrict__ dst,
+                                     const uint32_t* __restrict__ bits, uint32_t words, int x0, int y0,
+                                     T (*__restrict__ lds)[kRegionCells<BW, BH>])
 {
-    constexpr int RW = BW + 2, RH = BH + 2, RC = RW * RH;  // step-1 region
+    constexpr int RW = BW + 2, RC = kRegionCells<BW, BH>;
     constexpr int kIter1 = (RC + BS - 1) / BS, kIter2 = (BW * BH) / BS;
     static_assert((BW * BH) % BS == 0, "tile must be a multiple of the block");
-    __shared__ T lds[9][RC];
-    const int x0 = (int)(blockIdx.x % tiles_x) * BW;
-    const int y0 = (int)(blockIdx.x / tiles_x) * BH;
     const int nx = a.nx, ny = a.ny;
-    // Step 1: region cells (q, r) = (x0 - 1 + q, y0 - 1 + r), wrapped.
+    // Step 1: region cells (q, r) = (x0 - 1 + q, y0 - 1 + r), y wrapped, x
+    // wrapped when periodic.
 #pragma unroll
     for (int m = 0; m < kIter1; ++m) {
         const int k = (int)threadIdx.x + BS * m;
         if (k >= RC) break;
         const int r = k / RW, q = k % RW;
         int x = x0 - 1 + q, y = y0 - 1 + r;
+        if (x > nx || y > ny) continue;                  // beyond the ring of a cut tile
+        if (XEDGE == 1 && (x < 0 || x >= nx)) continue;  // outside the channel: never read
         x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
         y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
         const CellNbr nb = cell_nbr(a, bits, words, x, y);
         if (nb.s0 & 2u) continue;  // solid: never read by step 2
         float g[9];
-        cell_gather<T, 0, FLAGS>(a, src, nb, x, y, g);
+        cell_gather<T, XEDGE, FLAGS>(a, src, nb, x, y, g);
         float rho, ux, uy;
         phys_moments<FORCE>(a, g, rho, ux, uy);
         phys_collide<FORCE>(a, g, rho, ux, uy);
@@ -577,24 +586,54 @@ __global__ __launch_bounds__(BS) void k_phys_step2(PhysArgs a, const T* __restri
         const int k = (int)threadIdx.x + BS * m;
         const int row = k / BW, col = k % BW;
         const int x = x0 + col, y = y0 + row;
+        if (x >= nx || y >= ny) continue;
         const CellNbr nb = cell_nbr(a, bits, words, x, y);
         if (nb.s0 & 2u) continue;
-        const int c = (row + 1) * RW + (col + 1);
+        // gathering cell: the outlet column gathers as its west neighbour
+        int cx = x;
+        CellNbr gb = nb;
+        if (XEDGE == 1 && x == nx - 1 && !(nb.s0 & 1u)) {
+            cx = x - 1;
+            gb = cell_nbr(a, bits, words, cx, y);
+        }
+        const int c = (row + 1) * RW + (cx - x0 + 1);
         float g[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             const int ex = ex_of(i), ey = ey_of(i);
-            // source cell x - e_i: bit (1 - ex) of row y - ey
-            const uint32_t srow = ey == 1 ? nb.sm : (ey == -1 ? nb.sp : nb.s0);
-            const bool from_solid = (srow >> (1 - ex)) & 1u;
-            const T v = from_solid ? lds[inv_of(i)][c] : lds[i][c - ey * RW - ex];
+            // source cell cx - e_i: bit (1 - ex) of row y - ey
+            const uint32_t srow = ey == 1 ? gb.sm : (ey == -1 ? gb.sp : gb.s0);
+            bool own = (srow >> (1 - ex)) & 1u;
+            if (XEDGE == 1 && ex != 0) own = own || cx - ex < 0 || cx - ex >= nx;
+            const T v = own ? lds[inv_of(i)][c] : lds[i][c - ey * RW - ex];
             g[i] = Store<T>::from_storage(v, a.inv_scale);
+        }
+        if (XEDGE == 1 && x == 0) {
+            const float u0 = a.inlet_ux;
+            const float G = g[0] + g[2] + g[4] + 2.0f * (g[3] + g[6] + g[7]);
+            const float rho = (1.0f + G) / (1.0f - u0);
+            const float ru = rho * u0;
+            const float h = 0.5f * (g[4] - g[2]);
+            g[1] = g[3] + (2.0f / 3.0f) * ru;
+            g[5] = g[7] + h + (1.0f / 6.0f) * ru;
+            g[8] = g[6] - h + (1.0f / 6.0f) * ru;
         }
         float rho, ux, uy;
         phys_moments<FORCE>(a, g, rho, ux, uy);
         phys_collide<FORCE>(a, g, rho, ux, uy);
-        cell_store<T, 0, FLAGS>(a, dst, nb, x, y, g);
+        cell_store<T, XEDGE, FLAGS>(a, dst, nb, x, y, g);
     }
+}
+
+template <typename T, int XEDGE, bool FORCE, int BS, int BW, int BH, int FLAGS>
+__global__ __launch_bounds__(BS) void k_phys_step2(PhysArgs a, const T* __restrict__ src, T* __restrict__ dst,
+                                                   const uint32_t* __restrict__ bits, uint32_t words,
+                                                   uint32_t tiles_x)
+{
+    __shared__ T lds[9][kRegionCells<BW, BH>];
+    const int x0 = (int)(blockIdx.x % tiles_x) * BW;
+    const int y0 = (int)(blockIdx.x / tiles_x) * BH;
+    two_step_tile<T, XEDGE, FORCE, BS, BW, BH, FLAGS>(a, src, dst, bits, words, x0, y0, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -649,6 +688,33 @@ __global__ __launch_bounds__(BS) void k_batch_step(const BatchDomainDev* __restr
             if (D.force) batch_chunk<T, 0, true, BS, V, FLAGS>(D, t, src, dst, bits);
             else batch_chunk<T, 0, false, BS, V, FLAGS>(D, t, src, dst, bits);
         }
+    }
+}
+
+// Two steps of every domain: block t takes tile t (after the XCD remap when
+// kXcdRemap), of domain tile_dom[t], through the same tile function as the
+// one-domain two-step kernel.
+template <typename T, int BS, int BW, int BH, int FLAGS>
+__global__ __launch_bounds__(BS) void k_batch_step2(const BatchDomainDev* __restrict__ doms,
+                                                    const uint16_t* __restrict__ tile_dom, const T* __restrict__ src,
+                                                    T* __restrict__ dst, const uint32_t* __restrict__ bits)
+{
+    __shared__ T lds[9][kRegionCells<BW, BH>];
+    const uint32_t t = (FLAGS & kXcdRemap) ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const BatchDomainDev D = doms[tile_dom[t]];
+    const uint32_t local = t - D.tile0;
+    const uint32_t tiles_x = ((uint32_t)D.a.nx + (uint32_t)(BW - 1)) / (uint32_t)BW;
+    const int x0 = (int)(local % tiles_x) * BW, y0 = (int)(local / tiles_x) * BH;
+    const T* s = src + D.pop_off;
+    T* o = dst + D.pop_off;
+    const uint32_t* m = bits + D.bits_off;
+    constexpr int F = FLAGS & ~kXcdRemap;
+    if (D.x_edge) {
+        if (D.force) two_step_tile<T, 1, true, BS, BW, BH, F>(D.a, s, o, m, D.words, x0, y0, lds);
+        else two_step_tile<T, 1, false, BS, BW, BH, F>(D.a, s, o, m, D.words, x0, y0, lds);
+    } else {
+        if (D.force) two_step_tile<T, 0, true, BS, BW, BH, F>(D.a, s, o, m, D.words, x0, y0, lds);
+        else two_step_tile<T, 0, false, BS, BW, BH, F>(D.a, s, o, m, D.words, x0, y0, lds);
     }
 }
 
@@ -1395,21 +1461,25 @@ static hipError_t launch_phys_variant_shfl(const PhysArgs& a, const void* src, v
     }
 }
 
+// Whether the two-step tiles of width bw fit an nx x ny lattice: the ring
+// wraps in y (and in x when periodic), so nx, ny >= 3, and with the
+// inlet/outlet edge the last tile of a row holds at least two columns (the
+// outlet gathers through its west neighbour's ring).
+static bool two_step_fits(int bw, int nx, int ny, int xedge)
+{
+    return nx >= 3 && ny >= 3 && (xedge == 0 || nx % bw != 1);
+}
+
 template <typename T, int XEDGE, bool FORCE, int BS, int BW, int BH, int FLAGS>
 static hipError_t launch_phys_variant2(const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
                                        uint32_t words, hipStream_t s)
 {
-    if constexpr (XEDGE != 0) {
-        return hipErrorInvalidValue;
-    } else {
-        // whole tiles only; the one-cell ring wraps, so nx, ny >= 3
-        if (a.nx % BW != 0 || a.ny % BH != 0 || a.nx < 3 || a.ny < 3) return hipErrorInvalidValue;
-        const uint32_t tiles_x = (uint32_t)(a.nx / BW);
-        const uint32_t tiles = tiles_x * (uint32_t)(a.ny / BH);
-        hipLaunchKernelGGL((k_phys_step2<T, FORCE, BS, BW, BH, FLAGS>), dim3(tiles), dim3(BS), 0, s, a, (const T*)src,
-                           (T*)dst, bits, words, tiles_x);
-        return hipGetLastError();
-    }
+    if (!two_step_fits(BW, a.nx, a.ny, XEDGE)) return hipErrorInvalidValue;
+    const uint32_t tiles_x = (uint32_t)((a.nx + BW - 1) / BW);
+    const uint32_t tiles = tiles_x * (uint32_t)((a.ny + BH - 1) / BH);
+    hipLaunchKernelGGL((k_phys_step2<T, XEDGE, FORCE, BS, BW, BH, FLAGS>), dim3(tiles), dim3(BS), 0, s, a,
+                       (const T*)src, (T*)dst, bits, words, tiles_x);
+    return hipGetLastError();
 }
 
 // Tile shapes the step kernel is built for: threads per block, cells per
@@ -1452,8 +1522,7 @@ bool step_variant_ok(int v, int nx, int ny, int xedge)
 {
     if (v < 0 || v >= kNumStepVariants) return false;
     const StepVariant& sv = kStepVariants[v];
-    if (sv.flags & kTwoStep)
-        return xedge == 0 && nx % sv.vec == 0 && ny % sv.cells_per_thread == 0 && nx >= 3 && ny >= 3;
+    if (sv.flags & kTwoStep) return two_step_fits(sv.vec, nx, ny, xedge);
     if (sv.flags & kShfl) return xedge == 0 && nx % (sv.block * sv.vec) == 0;
     return sv.vec == 1 || nx % sv.vec == 0;
 }
@@ -1516,12 +1585,15 @@ static hipError_t dispatch_variant2(int v, const PhysArgs& a, const void* src, v
 hipError_t launch_phys_step2(int variant, bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
                              void* dst, const uint32_t* bits, uint32_t words, hipStream_t s)
 {
-    if (xedge) return hipErrorInvalidValue;
-    if (f16)
-        return force ? dispatch_variant2<_Float16, 0, true>(variant, a, src, dst, bits, words, s)
-                     : dispatch_variant2<_Float16, 0, false>(variant, a, src, dst, bits, words, s);
-    return force ? dispatch_variant2<float, 0, true>(variant, a, src, dst, bits, words, s)
-                 : dispatch_variant2<float, 0, false>(variant, a, src, dst, bits, words, s);
+#define LBM_D2(T, XE, FO) return dispatch_variant2<T, XE, FO>(variant, a, src, dst, bits, words, s)
+    if (f16) {
+        if (xedge) { if (force) LBM_D2(_Float16, 1, true); else LBM_D2(_Float16, 1, false); }
+        else { if (force) LBM_D2(_Float16, 0, true); else LBM_D2(_Float16, 0, false); }
+    }
+    if (xedge) { if (force) LBM_D2(float, 1, true); else LBM_D2(float, 1, false); }
+    if (force) LBM_D2(float, 0, true);
+    LBM_D2(float, 0, false);
+#undef LBM_D2
 }
 
 template <typename T, int XEDGE>
@@ -1553,6 +1625,10 @@ const BatchVariant kBatchVariants[] = {
     {64, 4, 16, kNtStore}, {128, 4, 8, 0},
     {256, 1, 8, kNtStore | kGridStride}, {512, 1, 4, kNtStore | kGridStride}, {1024, 1, 2, kNtStore | kGridStride},
     {256, 1, 16, kNtStore | kGridStride},
+    // two steps per launch: (block, tile width, -, flags, tile rows)
+    {256, 64, 0, kNtStore | kTwoStep, 16}, {512, 64, 0, kNtStore | kTwoStep, 16},
+    {512, 128, 0, kNtStore | kTwoStep, 8}, {256, 32, 0, kNtStore | kTwoStep, 16},
+    {256, 64, 0, kNtStore | kTwoStep, 8}, {256, 64, 0, kNtStore | kTwoStep | kXcdRemap, 16},
 };
 const int kNumBatchVariants = (int)(sizeof(kBatchVariants) / sizeof(kBatchVariants[0]));
 
@@ -1588,6 +1664,34 @@ hipError_t launch_batch_step(int variant, bool f16, const BatchDomainDev* doms, 
     if (nchunks == 0) return hipSuccess;
     return f16 ? launch_batch_typed<_Float16>(variant, doms, chunk_dom, nchunks, grid, src, dst, bits, s)
                : launch_batch_typed<float>(variant, doms, chunk_dom, nchunks, grid, src, dst, bits, s);
+}
+
+template <typename T>
+static hipError_t launch_batch2_typed(int v, const BatchDomainDev* doms, const uint16_t* tile_dom, uint32_t ntiles,
+                                      const void* src, void* dst, const uint32_t* bits, hipStream_t s)
+{
+#define LBM_BATCH2(BS, BW, BH, FL)                                                                            \
+    hipLaunchKernelGGL((k_batch_step2<T, BS, BW, BH, FL>), dim3(ntiles), dim3(BS), 0, s, doms, tile_dom,   \
+                       (const T*)src, (T*)dst, bits)
+    switch (v) {
+    case 10: LBM_BATCH2(256, 64, 16, kNtStore); break;
+    case 11: LBM_BATCH2(512, 64, 16, kNtStore); break;
+    case 12: LBM_BATCH2(512, 128, 8, kNtStore); break;
+    case 13: LBM_BATCH2(256, 32, 16, kNtStore); break;
+    case 14: LBM_BATCH2(256, 64, 8, kNtStore); break;
+    case 15: LBM_BATCH2(256, 64, 16, kNtStore | kXcdRemap); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef LBM_BATCH2
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_step2(int variant, bool f16, const BatchDomainDev* doms, const uint16_t* tile_dom,
+                              uint32_t ntiles, const void* src, void* dst, const uint32_t* bits, hipStream_t s)
+{
+    if (ntiles == 0) return hipSuccess;
+    return f16 ? launch_batch2_typed<_Float16>(variant, doms, tile_dom, ntiles, src, dst, bits, s)
+               : launch_batch2_typed<float>(variant, doms, tile_dom, ntiles, src, dst, bits, s);
 }
 
 hipError_t launch_restrict4(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, const uint8_t* mat,

@@ -110,6 +110,7 @@ _SIGNATURES = [
     ("lbm_batch_macro", C.c_int, [_P, C.c_int, _P, _P, _P]),
     ("lbm_batch_reduce", C.c_int, [_P, C.c_int, _P]),
     ("lbm_batch_variant_count", C.c_int, []),
+    ("lbm_batch_variant_info", C.c_int, [C.c_int] + [C.POINTER(C.c_int)] * 4),
     ("lbm_batch_set_variant", C.c_int, [_P, C.c_int]),
     ("lbm_batch_get_variant", C.c_int, [_P, C.POINTER(C.c_int)]),
     ("lbm_batch_set_graph_chunk", C.c_int, [_P, C.c_int]),
@@ -209,13 +210,13 @@ def variant_shapes() -> list[tuple[int, int, int, int]]:
 
 def variant_ok(variant: int, nx: int, ny: int, x_edge: int = X_PERIODIC) -> bool:
     """Whether a tile variant can step an nx x ny lattice (the library's
-    own rule: two-step variants need periodic x and whole tiles of
-    (rows, width) = (cells_per_thread, vec); lane-shuffle variants need
-    periodic x and whole blocks of block*vec cells per row; vector variants
-    need nx % vec == 0)."""
+    own rule: two-step variants need nx, ny >= 3 and, with the inlet/outlet
+    edge, a last tile of at least two columns (nx % vec != 1); lane-shuffle
+    variants need periodic x and whole blocks of block*vec cells per row;
+    vector variants need nx % vec == 0)."""
     b, c, v, f = variant_shapes()[variant]
     if f & 16:
-        return x_edge == X_PERIODIC and nx % v == 0 and ny % c == 0 and nx >= 3 and ny >= 3
+        return nx >= 3 and ny >= 3 and (x_edge == X_PERIODIC or nx % v != 1)
     if f & 8:
         return x_edge == X_PERIODIC and nx % (b * v) == 0
     return v == 1 or nx % v == 0
@@ -595,6 +596,20 @@ class Network:
 
 def batch_variant_count() -> int:
     return load_library().lbm_batch_variant_count()
+
+
+def batch_variant_shapes() -> list[tuple[int, int, int, int]]:
+    """(threads per block, consecutive cells per lane or two-step tile
+    width, two-step tile rows (0 for one-step variants), flags: 1
+    non-temporal stores, 4 work grouped per XCD, 16 two steps per launch,
+    32 persistent blocks take every grid-th chunk)."""
+    lib = load_library()
+    out = []
+    for v in range(lib.lbm_batch_variant_count()):
+        b, w, h, f = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(lib.lbm_batch_variant_info(v, C.byref(b), C.byref(w), C.byref(h), C.byref(f)))
+        out.append((b.value, w.value, h.value, f.value))
+    return out
 
 
 def exported_symbols() -> list[str]:

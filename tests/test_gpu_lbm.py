@@ -12,6 +12,7 @@ import pytest
 from oracle import oracle as O
 from simuverse_amd import lbm as L
 from simuverse_amd import workloads as W
+from visible import visible_mask
 
 pytestmark = pytest.mark.gpu
 
@@ -20,24 +21,6 @@ pytestmark = pytest.mark.gpu
 def gfx950():
     n = L.device_count()
     assert n >= 1, "no gfx950 device visible"
-
-
-def visible_mask(solid: np.ndarray, periodic_x: bool = True) -> np.ndarray:
-    """(9, ny, nx) mask of the population slots a fluid cell can ever read:
-    every slot of a fluid cell, and slot k of a solid cell S when the cell
-    S + e_k is fluid (it pulls direction k from S); y is periodic, x only
-    when periodic_x."""
-    ny, nx = solid.shape
-    vis = np.zeros((9, ny, nx), bool)
-    fluid = ~solid
-    for k in range(9):
-        pulls_from_me = np.roll(np.roll(fluid, -W.EX[k], axis=1), -W.EY[k], axis=0)
-        if not periodic_x and W.EX[k] == 1:
-            pulls_from_me[:, nx - 1] = False
-        if not periodic_x and W.EX[k] == -1:
-            pulls_from_me[:, 0] = False
-        vis[k] = fluid | (solid & pulls_from_me)
-    return vis
 
 
 def rel_l2(a, b, mask=None):
@@ -262,30 +245,51 @@ def test_physics_reduce_identities_small():
     assert abs(lat.reduce()[0] - m0) < 1e-6 * m0
 
 
+def edge_channel_solid(nx, ny):
+    """Walls on rows 0 and ny-1 and disks touching the inlet column, the
+    middle and the outlet columns (so that the outlet column has both fluid
+    and solid west neighbours)."""
+    y, x = np.mgrid[0:ny, 0:nx]
+    solid = (y == 0) | (y == ny - 1)
+    for cx, cy, r in ((nx // 3, ny // 2, 9), (nx - 3, ny // 3, 5), (1, 2 * ny // 3, 4)):
+        solid |= (x - cx) ** 2 + (y - cy) ** 2 <= r * r
+    return solid
+
+
 @pytest.mark.parametrize("storage", [L.STORAGE_F32, L.STORAGE_F16_DEV], ids=["f32", "f16"])
-@pytest.mark.parametrize("nx, ny", [(300, 200), (2048, 40), (512, 64)])
-def test_tile_variants_and_graphs_bit_identical(storage, nx, ny):
+@pytest.mark.parametrize("nx, ny, x_edge", [(300, 200, 0), (2048, 40, 0), (512, 64, 0), (300, 200, 1),
+                                            (258, 72, 1)])
+def test_tile_variants_and_graphs_bit_identical(storage, nx, ny, x_edge):
     """Every tile variant that can step the lattice, with and without
     graphs, in both device layouts (row pitch 304 > nx = 300 in the
     row-interleaved one) gives the same populations bit for bit; the
-    2048-wide lattice admits the lane-shuffle variants and the 512 x 64 one
-    the two-step (LDS tile) variants.  202 + 1 steps with graph chunks of 0,
-    100 or 102 steps (51 two-step launches: an odd number of buffer flips
-    per graph) and a final single step (the one-step fallback).  Compared at
-    every slot a fluid cell can read: a two-step launch ends in the other
-    buffer, whose solid-cell slots that no fluid cell pulls from hold older
+    2048-wide lattice admits the lane-shuffle variants, and every lattice
+    the two-step (LDS tile) variants, with tiles cut by the lattice edge
+    where nx or ny is not a multiple of the tile (258 leaves a two-column
+    last tile for every tile width).  The x_edge 1 cases are channels with
+    the Zou-He inlet and the zero-gradient outlet, and disks on the inlet
+    and outlet columns.  202 + 1 steps with graph chunks of 0, 100 or 102
+    steps (51 two-step launches: an odd number of buffer flips per graph)
+    and a final single step (the one-step fallback).  Compared at every
+    slot a fluid cell can read: a two-step launch ends in the other buffer,
+    whose solid-cell slots that no fluid cell pulls from hold older
     values."""
-    print(f"case: variants + graphs {nx}x{ny} storage {storage}", flush=True)
-    scene = W.periodic_disks(nx, ny, 0.05, 4, 10, seed=4)
-    f0 = W.rest_state(nx, ny, scene.solid, amplitude=1e-3).astype(np.float32)
+    print(f"case: variants + graphs {nx}x{ny} x_edge {x_edge} storage {storage}", flush=True)
+    if x_edge:
+        solid = edge_channel_solid(nx, ny)
+        kw = dict(x_edge=L.X_INLET_OUTLET, inlet_ux=0.05, force=(2e-6, 0.0))
+    else:
+        solid = W.periodic_disks(nx, ny, 0.05, 4, 10, seed=4).solid
+        kw = dict(force=(1e-5, 0.0))
+    f0 = W.rest_state(nx, ny, solid, amplitude=1e-3).astype(np.float32)
     outs, names = [], []
     nv = len(L.variant_shapes())
     for layout in (L.LAYOUT_ROWS, L.LAYOUT_PLANES):
         for v in range(nv):
-            if not L.variant_ok(v, nx, ny):
+            if not L.variant_ok(v, nx, ny, x_edge):
                 continue
-            lat = L.Lattice(nx, ny, 0.58, force=(1e-5, 0.0), layout=layout, storage=storage)
-            lat.write_materials(W.materials_from_solid(scene.solid))
+            lat = L.Lattice(nx, ny, 0.58, layout=layout, storage=storage, **kw)
+            lat.write_materials(W.materials_from_solid(solid))
             lat.write_populations(f0, 0)
             lat.set_variant(v)
             lat.set_graph_chunk((0, 100, 102)[v % 3])
@@ -296,9 +300,9 @@ def test_tile_variants_and_graphs_bit_identical(storage, nx, ny):
             lat.close()
     if nx % 1024 == 0:
         assert any(n[2][3] & 8 for n in names), "no lane-shuffle variant ran"
-    if nx == 512:
-        assert sum(1 for n in names if n[2][3] & 16) >= 6, "two-step variants did not run"
-    vis = visible_mask(scene.solid)
+    assert sum(1 for n in names if n[2][3] & 16) >= 2 * 8, "two-step variants did not run"
+    assert np.isfinite(outs[0]).all()
+    vis = visible_mask(solid, periodic_x=not x_edge)
     for k, o in enumerate(outs[1:], 1):
         bad = np.count_nonzero(o[vis] != outs[0][vis])
         assert bad == 0, f"variant {names[k]} differs from {names[0]} at {bad} visible slots"

@@ -1,0 +1,23 @@
+"""Population slots a fluid cell can read: the contract under which every
+tile shape of the step kernels, one or two steps per launch, agrees."""
+import numpy as np
+
+from simuverse_amd import workloads as W
+
+
+def visible_mask(solid: np.ndarray, periodic_x: bool = True) -> np.ndarray:
+    """(9, ny, nx) mask of the population slots a fluid cell can ever read:
+    every slot of a fluid cell, and slot k of a solid cell S when the cell
+    S + e_k is fluid (it pulls direction k from S); y is periodic, x only
+    when periodic_x."""
+    ny, nx = solid.shape
+    vis = np.zeros((9, ny, nx), bool)
+    fluid = ~solid
+    for k in range(9):
+        pulls_from_me = np.roll(np.roll(fluid, -W.EX[k], axis=1), -W.EY[k], axis=0)
+        if not periodic_x and W.EX[k] == 1:
+            pulls_from_me[:, nx - 1] = False
+        if not periodic_x and W.EX[k] == -1:
+            pulls_from_me[:, 0] = False
+        vis[k] = fluid | (solid & pulls_from_me)
+    return vis
