@@ -115,8 +115,9 @@ def run_copy_calibration(nbytes: int):
 
 def run_batch(args):
     """512 seeded uneven channel domains (sizes 64..1024, own tau, inlet ux,
-    0..8 disks) stepped by one persistent launch per step in 100-step HIP
-    graphs; aggregate MLUPS over all cells."""
+    0..8 disks) stepped by one launch per step (persistent chunks) or per
+    two steps (one block per LDS tile), whichever shape the autotuner picks,
+    in 100-step HIP graphs; aggregate MLUPS over all cells."""
     from simuverse_amd import lbm as L
     from simuverse_amd import workloads as W
 
@@ -132,7 +133,7 @@ def run_batch(args):
     if args.variant >= 0:
         b.set_variant(args.variant)
     else:
-        v, ms = b.autotune(steps=5)
+        v, ms = b.autotune(steps=40)
         log(f"batch autotune: variant {v} {ms:.3f} ms/step")
     b.run(args.warmup)
     b.synchronize()
@@ -142,6 +143,10 @@ def run_batch(args):
     step_s = ms / 1e3 / K
     bpc = 36 if args.f16 else 72
     achieved = bpc * cells / step_s / 1e9
+    variant = b.variant()
+    block, vec, tile_h, flags = L.batch_variant_shapes()[variant]
+    per_launch = 2 if flags & 16 else 1
+    workload = f"d2q9_batch512_{'f16dev' if args.f16 else 'f32'}"
     line = {
         "metric": METRIC,
         "value": round(cells / step_s / 1e6, 1),
@@ -155,10 +160,13 @@ def run_batch(args):
         "vs_baseline": None,
         "dtype": "f16-deviation storage, f32 math" if args.f16 else "f32",
         "data": "synthetic",
-        "config": {"workload": "d2q9_batch_512_domains", "domains": len(doms), "cells": int(cells),
-                   "variant": b.variant(), "graph_chunk": args.graph_chunk, "seed": W.SEED},
+        "config": {"workload": workload, "domains": len(doms), "cells": int(cells), "variant": variant,
+                   "batch_variant": {"kernel": "k_batch_step2" if per_launch == 2 else "k_batch_step",
+                                     "block": block, "vec": vec, "tile_rows": tile_h, "flags": flags},
+                   "graph_chunk": args.graph_chunk, "seed": W.SEED},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(workload, variant),
+                     "steps_per_launch": per_launch, "algorithmic_bytes_per_launch": bpc * cells * per_launch},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

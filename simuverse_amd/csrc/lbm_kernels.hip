@@ -631,9 +631,12 @@ __global__ __launch_bounds__(BS) void k_phys_step2(PhysArgs a, const T* __restri
                                                    uint32_t tiles_x)
 {
     __shared__ T lds[9][kRegionCells<BW, BH>];
-    const int x0 = (int)(blockIdx.x % tiles_x) * BW;
-    const int y0 = (int)(blockIdx.x / tiles_x) * BH;
-    two_step_tile<T, XEDGE, FORCE, BS, BW, BH, FLAGS>(a, src, dst, bits, words, x0, y0, lds);
+    // with kXcdRemap the blocks of one XCD take a contiguous range of tiles,
+    // so the ring rows and columns they share are read once into its L2
+    const uint32_t t = (FLAGS & kXcdRemap) ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int x0 = (int)(t % tiles_x) * BW;
+    const int y0 = (int)(t / tiles_x) * BH;
+    two_step_tile<T, XEDGE, FORCE, BS, BW, BH, FLAGS & ~kXcdRemap>(a, src, dst, bits, words, x0, y0, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1515,6 +1518,11 @@ const StepVariant kStepVariants[] = {
     {256, 8, 64, kNtStore | kTwoStep},
     {512, 32, 64, kNtStore | kTwoStep},
     {256, 16, 32, kNtStore | kTwoStep},
+    {512, 8, 128, kNtStore | kTwoStep | kXcdRemap},
+    {256, 16, 64, kNtStore | kTwoStep | kXcdRemap},
+    {512, 16, 64, kNtStore | kTwoStep | kXcdRemap},
+    {1024, 16, 64, kNtStore | kTwoStep | kXcdRemap},
+    {256, 8, 128, kNtStore | kTwoStep | kXcdRemap},
 };
 const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
 
@@ -1566,7 +1574,7 @@ template <typename T, int XEDGE, bool FORCE>
 static hipError_t dispatch_variant2(int v, const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
                                     uint32_t words, hipStream_t s)
 {
-    constexpr int S = kNtStore;
+    constexpr int S = kNtStore, R = kXcdRemap;
     switch (v) {
     case 18: return launch_phys_variant2<T, XEDGE, FORCE, 256, 128, 8, S>(a, src, dst, bits, words, s);
     case 19: return launch_phys_variant2<T, XEDGE, FORCE, 256, 64, 16, S>(a, src, dst, bits, words, s);
@@ -1578,6 +1586,11 @@ static hipError_t dispatch_variant2(int v, const PhysArgs& a, const void* src, v
     case 25: return launch_phys_variant2<T, XEDGE, FORCE, 256, 64, 8, S>(a, src, dst, bits, words, s);
     case 26: return launch_phys_variant2<T, XEDGE, FORCE, 512, 64, 32, S>(a, src, dst, bits, words, s);
     case 27: return launch_phys_variant2<T, XEDGE, FORCE, 256, 32, 16, S>(a, src, dst, bits, words, s);
+    case 28: return launch_phys_variant2<T, XEDGE, FORCE, 512, 128, 8, S | R>(a, src, dst, bits, words, s);
+    case 29: return launch_phys_variant2<T, XEDGE, FORCE, 256, 64, 16, S | R>(a, src, dst, bits, words, s);
+    case 30: return launch_phys_variant2<T, XEDGE, FORCE, 512, 64, 16, S | R>(a, src, dst, bits, words, s);
+    case 31: return launch_phys_variant2<T, XEDGE, FORCE, 1024, 64, 16, S | R>(a, src, dst, bits, words, s);
+    case 32: return launch_phys_variant2<T, XEDGE, FORCE, 256, 128, 8, S | R>(a, src, dst, bits, words, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1629,6 +1642,8 @@ const BatchVariant kBatchVariants[] = {
     {256, 64, 0, kNtStore | kTwoStep, 16}, {512, 64, 0, kNtStore | kTwoStep, 16},
     {512, 128, 0, kNtStore | kTwoStep, 8}, {256, 32, 0, kNtStore | kTwoStep, 16},
     {256, 64, 0, kNtStore | kTwoStep, 8}, {256, 64, 0, kNtStore | kTwoStep | kXcdRemap, 16},
+    {512, 64, 0, kNtStore | kTwoStep | kXcdRemap, 16}, {512, 128, 0, kNtStore | kTwoStep | kXcdRemap, 8},
+    {256, 32, 0, kNtStore | kTwoStep | kXcdRemap, 16}, {1024, 64, 0, kNtStore | kTwoStep | kXcdRemap, 16},
 };
 const int kNumBatchVariants = (int)(sizeof(kBatchVariants) / sizeof(kBatchVariants[0]));
 
@@ -1680,6 +1695,10 @@ static hipError_t launch_batch2_typed(int v, const BatchDomainDev* doms, const u
     case 13: LBM_BATCH2(256, 32, 16, kNtStore); break;
     case 14: LBM_BATCH2(256, 64, 8, kNtStore); break;
     case 15: LBM_BATCH2(256, 64, 16, kNtStore | kXcdRemap); break;
+    case 16: LBM_BATCH2(512, 64, 16, kNtStore | kXcdRemap); break;
+    case 17: LBM_BATCH2(512, 128, 8, kNtStore | kXcdRemap); break;
+    case 18: LBM_BATCH2(256, 32, 16, kNtStore | kXcdRemap); break;
+    case 19: LBM_BATCH2(1024, 64, 16, kNtStore | kXcdRemap); break;
     default: return hipErrorInvalidValue;
     }
 #undef LBM_BATCH2
