@@ -186,9 +186,12 @@ def run_learned(args):
        goes into the network's sample store (inputs: non-equilibrium part of
        the coarse populations; target: the residual of one coarse BGK step);
     2. Adam training on those samples (binary32, device);
-    3. coarse runs of N_f / 4 steps from the restricted fine initial state,
-       plain BGK and BGK + the network on the matrix cores (timed);
-    4. velocity relative L2 error of each coarse run against the restricted
+    3. a validation gate: the gain on the correction (1, 1/2, 1/4 or 0, plain
+       BGK) with the smallest coarse velocity error at N_f / 2, the end of
+       the sampled half of the window;
+    4. coarse runs of N_f / 4 steps from the restricted fine initial state,
+       plain BGK and BGK + the gated network on the matrix cores (timed);
+    5. velocity relative L2 error of each coarse run against the restricted
        fine velocity at N_f, and time-to-solution ratios.
     """
     from simuverse_amd import lbm as L
@@ -237,6 +240,10 @@ def run_learned(args):
         b, _, _, _ = fine2.restrict4(tau_c, gc)
         net.add_transition(a, b, solid_c, tau_c, gc, stride=stride, offset=pairs % stride)
         pairs += 1
+    # the restricted fine velocity at N_f / 2: the validation target
+    if done <= Nf // 2:
+        fine2.run(Nf // 2 - done)
+    _, _, uxh, uyh = fine2.restrict4(tau_c, gc)
     fine2.close()
     rx, rt = net.normalize_samples()
     t_collect = time.perf_counter() - t0
@@ -247,7 +254,7 @@ def run_learned(args):
     log(f"samples {net.sample_count()} from {pairs} transitions ({t_collect:.1f} s), "
         f"train {args.learned_train_steps} steps ({t_train:.1f} s), loss {loss:.3e}")
 
-    def coarse(learned: bool):
+    def coarse(learned: bool, steps: int, ref_u):
         lat = L.Lattice(nc, nc, tau_c, force=gc)
         lat.write_materials(W.materials_from_solid(solid_c))
         lat.write_populations(fc0, 0)
@@ -256,18 +263,38 @@ def run_learned(args):
         else:
             lat.autotune(steps=10)
             lat.write_populations(fc0, 0)
-        ms = lat.run_timed(Nf // 4)
+        ms = lat.run_timed(steps)
         _, ux, uy = lat.macro()
         u, finite = lat.health()
         fl = ~solid_c
-        ref = np.hypot(uxr, uyr)[fl]
-        err = float(np.linalg.norm(np.hypot(ux - uxr, uy - uyr)[fl]) / np.linalg.norm(ref)) if finite else float("inf")
+        ur, vr = ref_u
+        ref = np.hypot(ur, vr)[fl]
+        err = float(np.linalg.norm(np.hypot(ux - ur, uy - vr)[fl]) / np.linalg.norm(ref)) if finite else float("inf")
         lat.close()
         return ms, err
 
-    ms_plain, e_plain = coarse(False)
-    ms_learn, e_learn = coarse(True)
-    log(f"coarse plain: {ms_plain:.1f} ms, vel L2 {e_plain:.4e}; learned: {ms_learn:.1f} ms, vel L2 {e_learn:.4e}")
+    # Validation gate: the correction's gain (a factor on the output scale)
+    # is chosen on the first half of the window, coarse runs to N_f / 2
+    # against the restricted fine velocity there; 0 is plain BGK.  The
+    # reported errors are at N_f, which no sample or gate run has seen.
+    s_in, s_out = net.scales()
+    gate = {}
+    for gain in (1.0, 0.5, 0.25, 0.0):
+        if gain > 0.0:
+            net.set_scales(s_in, s_out * gain)
+        gate[gain] = coarse(gain > 0.0, Nf // 8, (uxh, uyh))[1]
+    gain = min(gate, key=lambda k: (gate[k], -k))
+    log("validation gate (vel L2 at N_f/2 by gain): " + ", ".join(f"{k:g}: {v:.4e}" for k, v in gate.items())
+        + f"; chosen gain {gain:g}")
+
+    ms_plain, e_plain = coarse(False, Nf // 4, (uxr, uyr))
+    # the learned kernel at the chosen gain; at gain 0 the gate falls back to
+    # plain BGK and the learned kernel is timed at gain 1 for reference
+    net.set_scales(s_in, s_out * (gain if gain > 0.0 else 1.0))
+    ms_learn, e_learn = coarse(True, Nf // 4, (uxr, uyr))
+    e_gated, ms_gated = (e_learn, ms_learn) if gain > 0.0 else (e_plain, ms_plain)
+    log(f"coarse plain: {ms_plain:.1f} ms, vel L2 {e_plain:.4e}; learned (gain {gain if gain > 0 else 1:g}): {ms_learn:.1f} ms, "
+        f"vel L2 {e_learn:.4e}")
     K = Nf // 4
     line = {
         "metric": METRIC,
@@ -290,6 +317,9 @@ def run_learned(args):
             "training": {"pairs": pairs, "samples": net.sample_count(), "steps": args.learned_train_steps,
                          "batch": min(args.learned_batch, net.sample_count()), "final_loss": loss,
                          "collect_s": round(t_collect, 2), "train_s": round(t_train, 2)},
+            "validation_gate": {"vel_l2_at_half_window_by_gain": {f"{k:g}": v for k, v in gate.items()},
+                                "gain": gain, "vel_l2": e_gated, "coarse_ms": round(ms_gated, 3),
+                                "speedup_vs_fine": round(ms_fine / ms_gated, 2)},
             "vel_l2_vs_restricted_fine": {"plain": e_plain, "learned": e_learn},
             "coarse_ms": {"plain": round(ms_plain, 3), "learned": round(ms_learn, 3)},
             "time_to_solution_speedup": {"learned_vs_fine": round(ms_fine / ms_learn, 2),
