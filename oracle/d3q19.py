@@ -1,0 +1,125 @@
+"""Binary64 D3Q19 oracle (numpy).
+
+TEST INFRASTRUCTURE ONLY.  The tests and the smoke check use this module as
+the checker of the 3-D kernels; the product code in simuverse_amd/ never
+imports it.
+
+The reference declares a 3-D fluid simulator type (simuverse/src/lib.rs:114,
+`SimuType::D3Fluid`) but implements none, so there is no reference code to
+follow.  This is the textbook D3Q19 BGK lattice Boltzmann method (Qian,
+d'Humieres & Lallemand, Europhys. Lett. 17 (1992) 479) with the same
+conventions as the 2-D physics mode:
+
+  * the state is the post-collision populations of every fluid cell;
+  * a step gathers f_i(x) = f*_i(x - e_i), periodic in x, y and z, except
+    that a population that would come from a solid cell is the cell's own
+    f*_inv(i)(x) (half-way bounce-back on resting walls);
+  * rho = sum f, u = (sum e f + g/2) / rho, feq_i = w_i rho (1 + 3 e.u +
+    4.5 (e.u)^2 - 1.5 u^2), f* = f - (f - feq)/tau + F with Guo's forcing
+    F_i = (1 - 1/(2 tau)) w_i (3 (e_i - u).g + 9 (e_i.u)(e_i.g)) (Guo, Zheng
+    & Shi, Phys. Rev. E 65 (2002) 046308).
+
+Direction order: 0 rest; 1-6 faces (+x, -x, +y, -y, +z, -z); 7-18 edges;
+opposite directions are adjacent (inv(i) = i + 1 for odd i, i - 1 for even
+i >= 2).  Arrays are (19, nz, ny, nx); cell (x, y, z) is [.., z, y, x].
+
+Pinned in tests/test_oracle_d3q19.py by the lattice's moment identities, the
+rest-state fixed point, exact mass and momentum bookkeeping, the decay rate
+of a shear wave (nu = (tau - 1/2)/3) and the exact parabolic profile of
+body-force Poiseuille flow between half-way bounce-back walls at
+tau = 1/2 + sqrt(3/16).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+E = np.array([
+    (0, 0, 0),
+    (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+    (1, 1, 0), (-1, -1, 0), (1, -1, 0), (-1, 1, 0),
+    (1, 0, 1), (-1, 0, -1), (1, 0, -1), (-1, 0, 1),
+    (0, 1, 1), (0, -1, -1), (0, 1, -1), (0, -1, 1),
+], dtype=np.int64)
+W = np.array([1.0 / 3.0] + [1.0 / 18.0] * 6 + [1.0 / 36.0] * 12)
+INV = np.array([0] + [i + 1 if i % 2 == 1 else i - 1 for i in range(1, 19)], dtype=np.int64)
+
+
+def feq(rho: np.ndarray, u: np.ndarray) -> np.ndarray:
+    """Equilibrium (19, ...) of density rho (...) and velocity u (3, ...)."""
+    eu = np.tensordot(E.astype(np.float64), u, axes=(1, 0))
+    usq = (u * u).sum(0)
+    return W.reshape((19,) + (1,) * rho.ndim) * rho * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * usq)
+
+
+def rest_state(nx: int, ny: int, nz: int) -> np.ndarray:
+    return np.broadcast_to(W[:, None, None, None], (19, nz, ny, nx)).copy()
+
+
+class D3Q19State:
+    def __init__(self, tau: float, solid: np.ndarray, f: np.ndarray, force=(0.0, 0.0, 0.0)):
+        self.tau = float(tau)
+        self.solid = np.asarray(solid, bool)
+        self.f = np.array(f, np.float64)
+        self.g = np.asarray(force, np.float64)
+        self.f[:, self.solid] = 0.0
+        # solid[x - e_i] at each x, per direction
+        self._src_solid = np.stack([np.roll(self.solid, tuple(E[i][::-1]), axis=(0, 1, 2)) for i in range(19)])
+
+    def gather(self) -> np.ndarray:
+        """Pre-collision populations of every cell (solids: meaningless)."""
+        pre = np.empty_like(self.f)
+        for i in range(19):
+            pulled = np.roll(self.f[i], tuple(E[i][::-1]), axis=(0, 1, 2))
+            pre[i] = np.where(self._src_solid[i], self.f[INV[i]], pulled)
+        return pre
+
+    def moments(self, pre: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        rho = pre.sum(0)
+        j = np.tensordot(E.T.astype(np.float64), pre, axes=(1, 0))
+        u = (j + 0.5 * self.g[:, None, None, None]) / rho
+        return rho, u
+
+    def step(self) -> None:
+        pre = self.gather()
+        rho, u = self.moments(pre)
+        om = 1.0 / self.tau
+        post = pre - om * (pre - feq(rho, u))
+        if self.g.any():
+            eu = np.tensordot(E.astype(np.float64), u, axes=(1, 0))
+            eg = E.astype(np.float64) @ self.g
+            ug = np.tensordot(self.g, u, axes=(0, 0))
+            F = (1.0 - 0.5 * om) * W[:, None, None, None] * (
+                3.0 * (eg[:, None, None, None] - ug) + 9.0 * eu * eg[:, None, None, None])
+            post = post + F
+        post[:, self.solid] = 0.0
+        self.f = post
+
+    def run(self, nsteps: int) -> None:
+        for _ in range(nsteps):
+            self.step()
+
+    def macro(self) -> tuple[np.ndarray, np.ndarray]:
+        """rho (nz, ny, nx) and u (3, nz, ny, nx) of the gathered
+        populations (the next step's), zero at solid cells."""
+        rho, u = self.moments(self.gather())
+        rho = np.where(self.solid, 0.0, rho)
+        u = np.where(self.solid, 0.0, u)
+        return rho, u
+
+    def reduce(self) -> np.ndarray:
+        """[mass, momentum x/y/z, force on solids x/y/z]: sums over fluid
+        cells of the gathered populations and their first moments, and the
+        momentum exchange 2 e_i f*_i over every fluid-to-solid link."""
+        pre = self.gather()
+        fl = ~self.solid
+        out = np.zeros(7)
+        out[0] = pre[:, fl].sum()
+        for a in range(3):
+            out[1 + a] = (E[:, a, None] * pre[:, fl]).sum()
+        for i in range(1, 19):
+            # fluid x whose neighbour x + e_i is solid
+            dst_solid = np.roll(self.solid, tuple((-E[i])[::-1]), axis=(0, 1, 2))
+            link = fl & dst_solid
+            for a in range(3):
+                out[4 + a] += 2.0 * E[i, a] * self.f[i][link].sum()
+        return out

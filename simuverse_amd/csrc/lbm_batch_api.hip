@@ -22,9 +22,6 @@
 
 using namespace lbm;
 
-// Shared with lbm_api.hip: the thread-local error message.
-int lbm_fail(int code, const char* fmt, ...);
-
 #define BATCH_TRY(expr)                                                                               \
     do {                                                                                              \
         hipError_t e_ = (expr);                                                                       \

@@ -60,6 +60,26 @@ __device__ inline void st(T* p, T v)
     else *p = v;
 }
 
+// Packed solid mask: row y holds bit x+1 for cell x; bit 0 and bit nx+1
+// are the x-neighbours beyond the edges (the wrapped cells when x is
+// periodic, empty otherwise).  `words` per row is even and covers bit
+// nx+2, so the 64-bit window read below never leaves the row.
+// Returns bits (x-1, x, x+1) of row y as bits 0, 1, 2.
+__device__ inline uint32_t solid3(const uint32_t* __restrict__ bits, uint32_t words, int y, int x)
+{
+    const uint32_t* p = bits + (uint32_t)y * words + ((uint32_t)x >> 5);
+    const uint64_t v = (uint64_t)p[0] | ((uint64_t)p[1] << 32);
+    return (uint32_t)(v >> ((uint32_t)x & 31u)) & 7u;
+}
+
+// Sum over the 64 lanes of a wavefront (butterfly; every lane gets the sum).
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 // Step-kernel tuning flags.
 enum : int {
     kNtStore = 1,  // non-temporal population stores

@@ -6,6 +6,10 @@
 
 #include "lbm_device.hpp"
 
+// Records the message of a failure for lbm_last_error() and returns `code`
+// (defined in lbm_api.hip; every source file of the C ABI reports through it).
+int lbm_fail(int code, const char* fmt, ...);
+
 namespace lbm {
 
 struct StepVariant {

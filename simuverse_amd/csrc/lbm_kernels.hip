@@ -71,18 +71,6 @@ __device__ inline void phys_gather(const PhysArgs& a, const T* __restrict__ src,
     }
 }
 
-// Packed solid mask: row y holds bit x+1 for cell x; bit 0 and bit nx+1
-// are the x-neighbours beyond the edges (the wrapped cells when x is
-// periodic, empty otherwise).  `words` per row is even and covers bit
-// nx+2, so the 64-bit window read below never leaves the row.
-// Returns bits (x-1, x, x+1) of row y as bits 0, 1, 2.
-__device__ inline uint32_t solid3(const uint32_t* __restrict__ bits, uint32_t words, int y, int x)
-{
-    const uint32_t* p = bits + (uint32_t)y * words + ((uint32_t)x >> 5);
-    const uint64_t v = (uint64_t)p[0] | ((uint64_t)p[1] << 32);
-    return (uint32_t)(v >> ((uint32_t)x & 31u)) & 7u;
-}
-
 // Neighbourhood of a cell: the wrapped rows above and below, and the solid
 // flags (x-1, x, x+1) of rows ym, y, yp from three 64-bit windows of the
 // packed mask.
@@ -1189,13 +1177,6 @@ __global__ __launch_bounds__(256) void k_wgsl_reset(WgslArgs a, int fluid_ty, fl
 // ---------------------------------------------------------------------------
 // binary64 reductions: mass, momentum, momentum-exchange force.
 // ---------------------------------------------------------------------------
-
-__device__ inline double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 // Block of 256 threads: 5 sums -> partial[blockIdx.x*5 + k], fixed order.
 __device__ inline void block_sum5(double v[5], double* __restrict__ partial)

@@ -19,8 +19,6 @@
 
 using namespace lbm;
 
-int lbm_fail(int code, const char* fmt, ...);
-
 #define MLP_TRY(expr)                                                                               \
     do {                                                                                            \
         hipError_t e_ = (expr);                                                                     \

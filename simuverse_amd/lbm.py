@@ -137,7 +137,39 @@ _SIGNATURES = [
     ("lbm_health", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_int)]),
     ("lbm_run_guarded", C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_int)]),
     ("lbm_restrict4", C.c_int, [_P, C.c_float, C.c_float, C.c_float, _P, _P, _P, _P]),
+    # include/lbm3d.h
+    ("lbm3d_create", C.c_int, [_P, C.POINTER(_P)]),
+    ("lbm3d_destroy", C.c_int, [_P]),
+    ("lbm3d_write_materials", C.c_int, [_P, _P]),
+    ("lbm3d_reset", C.c_int, [_P]),
+    ("lbm3d_write_populations", C.c_int, [_P, _P]),
+    ("lbm3d_read_populations", C.c_int, [_P, _P]),
+    ("lbm3d_run", C.c_int, [_P, C.c_int]),
+    ("lbm3d_run_timed", C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
+    ("lbm3d_synchronize", C.c_int, [_P]),
+    ("lbm3d_macro", C.c_int, [_P, _P, _P, _P, _P]),
+    ("lbm3d_reduce", C.c_int, [_P, _P]),
+    ("lbm3d_variant_count", C.c_int, []),
+    ("lbm3d_variant_info", C.c_int, [C.c_int] + [C.POINTER(C.c_int)] * 3),
+    ("lbm3d_set_variant", C.c_int, [_P, C.c_int]),
+    ("lbm3d_get_variant", C.c_int, [_P, C.POINTER(C.c_int)]),
+    ("lbm3d_autotune", C.c_int, [_P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    ("lbm3d_set_graph_chunk", C.c_int, [_P, C.c_int]),
 ]
+
+
+class Desc3(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("nz", C.c_int32),
+        ("tau", C.c_float),
+        ("storage", C.c_int32),
+        ("force_x", C.c_float),
+        ("force_y", C.c_float),
+        ("force_z", C.c_float),
+        ("reserved", C.c_int32 * 8),
+    ]
 
 
 class DomainDesc(C.Structure):
@@ -592,6 +624,110 @@ class Network:
         loss = C.c_double()
         _check(self._lib.lbm_mlp_train(self._h, steps, batch, lr, seed, C.byref(loss)))
         return loss.value
+
+
+class Lattice3D:
+    """One D3Q19 lattice (periodic, bounce-back solids, Guo force) on the
+    current HIP device (include/lbm3d.h).  Arrays are (19, nz, ny, nx)
+    populations and (nz, ny, nx) fields."""
+
+    def __init__(self, nx: int, ny: int, nz: int, tau: float, storage: int = STORAGE_F32,
+                 force: tuple[float, float, float] = (0.0, 0.0, 0.0)):
+        self._lib = load_library()
+        self.shape = (int(nz), int(ny), int(nx))
+        self.n = int(nx) * int(ny) * int(nz)
+        d = Desc3()
+        d.nx, d.ny, d.nz, d.tau, d.storage = int(nx), int(ny), int(nz), tau, storage
+        d.force_x, d.force_y, d.force_z = force
+        h = C.c_void_p()
+        _check(self._lib.lbm3d_create(C.byref(d), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _check(self._lib.lbm3d_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def write_materials(self, material: np.ndarray) -> None:
+        m = np.ascontiguousarray(material, dtype=np.uint8).reshape(-1)
+        if m.size != self.n:
+            raise ValueError(f"material has {m.size} cells, lattice {self.n}")
+        _check(self._lib.lbm3d_write_materials(self._h, _ptr(m)))
+
+    def reset(self) -> None:
+        _check(self._lib.lbm3d_reset(self._h))
+
+    def write_populations(self, f: np.ndarray) -> None:
+        f = np.ascontiguousarray(f, dtype=np.float32).reshape(19, *self.shape)
+        _check(self._lib.lbm3d_write_populations(self._h, _ptr(f)))
+
+    def read_populations(self) -> np.ndarray:
+        out = np.empty((19,) + self.shape, np.float32)
+        _check(self._lib.lbm3d_read_populations(self._h, _ptr(out)))
+        return out
+
+    def run(self, nsteps: int) -> None:
+        _check(self._lib.lbm3d_run(self._h, nsteps))
+
+    def run_timed(self, nsteps: int) -> float:
+        ms = C.c_double()
+        _check(self._lib.lbm3d_run_timed(self._h, nsteps, C.byref(ms)))
+        return ms.value
+
+    def synchronize(self) -> None:
+        _check(self._lib.lbm3d_synchronize(self._h))
+
+    def macro(self) -> tuple[np.ndarray, np.ndarray]:
+        """rho (nz, ny, nx) and u (3, nz, ny, nx)."""
+        rho = np.empty(self.shape, np.float32)
+        u = np.empty((3,) + self.shape, np.float32)
+        _check(self._lib.lbm3d_macro(self._h, _ptr(rho), _ptr(u[0]), _ptr(u[1]), _ptr(u[2])))
+        return rho, u
+
+    def reduce(self) -> np.ndarray:
+        out = np.zeros(7, np.float64)
+        _check(self._lib.lbm3d_reduce(self._h, _ptr(out)))
+        return out
+
+    def set_variant(self, v: int) -> None:
+        _check(self._lib.lbm3d_set_variant(self._h, v))
+
+    def variant(self) -> int:
+        v = C.c_int()
+        _check(self._lib.lbm3d_get_variant(self._h, C.byref(v)))
+        return v.value
+
+    def set_graph_chunk(self, chunk: int) -> None:
+        _check(self._lib.lbm3d_set_graph_chunk(self._h, chunk))
+
+    def autotune(self, steps: int = 20) -> tuple[int, float]:
+        v, ms = C.c_int(), C.c_double()
+        _check(self._lib.lbm3d_autotune(self._h, steps, C.byref(v), C.byref(ms)))
+        return v.value, ms.value
+
+
+def variant3_shapes() -> list[tuple[int, int, int]]:
+    """(threads per block, cells per thread, flags: 1 non-temporal stores)
+    of the D3Q19 step kernel's shapes."""
+    lib = load_library()
+    out = []
+    for v in range(lib.lbm3d_variant_count()):
+        b, c, f = C.c_int(), C.c_int(), C.c_int()
+        _check(lib.lbm3d_variant_info(v, C.byref(b), C.byref(c), C.byref(f)))
+        out.append((b.value, c.value, f.value))
+    return out
 
 
 def batch_variant_count() -> int:

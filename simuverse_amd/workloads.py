@@ -241,6 +241,56 @@ def periodic_disks(nx: int, ny: int, fraction: float = 0.05, rmin: int = 4, rmax
 
 
 @dataclass
+class SphereScene:
+    nx: int
+    ny: int
+    nz: int
+    solid: np.ndarray          # (nz, ny, nx) bool
+    spheres: np.ndarray        # (k, 4) int: cx, cy, cz, r
+    seed: int
+
+    @property
+    def solid_fraction(self) -> float:
+        return float(self.solid.mean())
+
+
+def periodic_spheres(nx: int, ny: int, nz: int, fraction: float = 0.05, rmin: int = 4, rmax: int = 12,
+                     seed: int = SEED, max_tries: int = 1_000_000) -> SphereScene:
+    """Seeded non-overlapping spheres on a periodic 3-D lattice (the D3Q19
+    workload), the 3-D form of periodic_disks: a candidate (cx, cy, cz, r)
+    is drawn uniformly, rejected when its periodic centre distance to an
+    accepted sphere is <= r + r' + 1, and spheres are added until at least
+    `fraction` of the cells are solid."""
+    rng = _Stream(seed)
+    solid = np.zeros((nz, ny, nx), bool)
+    target = math.ceil(fraction * nx * ny * nz)
+    count, tries = 0, 0
+    cs = np.zeros((0, 3), np.int64)
+    rs = np.zeros(0, np.int64)
+    dims = np.array([nx, ny, nz])
+    while count < target:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("sphere placement did not reach the requested solid fraction")
+        c = np.array([int(rng.integers(0, n - 1)[0]) for n in (nx, ny, nz)])
+        r = int(rng.integers(rmin, rmax)[0])
+        if 2 * r + 1 > min(nx, ny, nz):
+            continue
+        if len(rs):
+            d = np.abs(cs - c)
+            d = np.minimum(d, dims - d)
+            if np.any((d * d).sum(1) <= (rs + r + 1) ** 2):
+                continue
+        oz, oy, ox = np.mgrid[-r:r + 1, -r:r + 1, -r:r + 1]
+        inside = ox * ox + oy * oy + oz * oz <= r * r
+        solid[(c[2] + oz[inside]) % nz, (c[1] + oy[inside]) % ny, (c[0] + ox[inside]) % nx] = True
+        count += int(inside.sum())
+        cs = np.vstack([cs, c])
+        rs = np.append(rs, r)
+    return SphereScene(nx, ny, nz, solid, np.hstack([cs, rs[:, None]]), seed)
+
+
+@dataclass
 class BatchDomain:
     nx: int
     ny: int

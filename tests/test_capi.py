@@ -62,9 +62,35 @@ def test_invalid_descriptors_are_rejected(kw, needle):
     assert ei.value.code in (1, 4)
 
 
+@pytest.mark.parametrize(
+    "kw, needle",
+    [
+        (dict(nx=2), ">= 3"),
+        (dict(nz=0), ">= 3"),
+        (dict(tau=0.5), "tau"),
+        (dict(storage=9), "storage"),
+        (dict(force=(float("nan"), 0.0, 0.0)), "finite"),
+        (dict(nx=4096, ny=4096, nz=4096), "32-bit"),
+    ],
+)
+def test_invalid_3d_descriptors_are_rejected(kw, needle):
+    args = dict(nx=8, ny=8, nz=8, tau=0.6)
+    args.update(kw)
+    with pytest.raises(L.LbmError) as ei:
+        L.Lattice3D(**args)
+    assert needle in str(ei.value)
+    assert ei.value.code == 1
+
+
+def test_3d_struct_layout():
+    assert C.sizeof(L.Desc3) == 64
+
+
 def test_null_handle_is_an_error_not_a_crash():
     lib = L.load_library()
     assert lib.lbm_step(None, 0, 0) == 1
+    assert b"null handle" in lib.lbm_last_error()
+    assert lib.lbm3d_run(None, 1) == 1
     assert b"null handle" in lib.lbm_last_error()
 
 

@@ -21,3 +21,15 @@ def visible_mask(solid: np.ndarray, periodic_x: bool = True) -> np.ndarray:
             pulls_from_me[:, 0] = False
         vis[k] = fluid | (solid & pulls_from_me)
     return vis
+
+
+def visible_mask3(solid: np.ndarray, e: np.ndarray) -> np.ndarray:
+    """(19, nz, ny, nx) mask of the D3Q19 slots a fluid cell can read (every
+    axis periodic); e is the (19, 3) direction table."""
+    fluid = ~solid
+    vis = np.empty((len(e),) + solid.shape, bool)
+    for k in range(len(e)):
+        # S + e_k fluid: S pulls nothing, but S + e_k pulls direction k from S
+        pulls_from_me = np.roll(fluid, tuple((-e[k])[::-1]), axis=(0, 1, 2))
+        vis[k] = fluid | (solid & pulls_from_me)
+    return vis
