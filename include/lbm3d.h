@@ -76,7 +76,8 @@ int lbm3d_reduce(lbm3d_handle* h, double out[7]);
 /* Step-kernel shapes: every shape computes bit-identical populations. */
 int lbm3d_variant_count(void);
 /* block: threads per block; cells_per_thread: cells of one row per thread
- * (strided by the block); flags: bit 0 non-temporal population stores. */
+ * (strided by the block); flags: bit 0 non-temporal population stores,
+ * bit 2 the blocks one XCD receives take a contiguous range of rows. */
 int lbm3d_variant_info(int variant, int* block, int* cells_per_thread, int* flags);
 int lbm3d_set_variant(lbm3d_handle* h, int variant);
 int lbm3d_get_variant(lbm3d_handle* h, int* variant);

@@ -200,9 +200,12 @@ template <typename T, bool FORCE, int BS, int CPT, int FLAGS>
 __global__ __launch_bounds__(BS) void k3_step(Args3 a, const T* __restrict__ src, T* __restrict__ dst,
                                               const uint32_t* __restrict__ bits, uint32_t words, uint32_t nseg)
 {
-    const uint32_t row = blockIdx.x / nseg;
+    // with kXcdRemap the blocks one XCD receives take a contiguous range of
+    // rows, so the neighbour rows a block gathers from sit in the same L2
+    const uint32_t b = (FLAGS & kXcdRemap) ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t row = b / nseg;
     const int y = (int)(row % (uint32_t)a.ny), z = (int)(row / (uint32_t)a.ny);
-    const int x0 = (int)(blockIdx.x % nseg) * (BS * CPT) + (int)threadIdx.x;
+    const int x0 = (int)(b % nseg) * (BS * CPT) + (int)threadIdx.x;
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int x = x0 + k * BS;
@@ -378,6 +381,7 @@ struct Variant3 {
 const Variant3 kVariants3[] = {
     {256, 1, kNtStore}, {256, 1, 0}, {128, 1, kNtStore}, {512, 1, kNtStore},
     {1024, 1, kNtStore}, {256, 2, kNtStore}, {64, 4, kNtStore},
+    {256, 1, kNtStore | kXcdRemap}, {512, 1, kNtStore | kXcdRemap}, {128, 1, kNtStore | kXcdRemap},
 };
 constexpr int kNumVariants3 = (int)(sizeof(kVariants3) / sizeof(kVariants3[0]));
 
@@ -404,6 +408,9 @@ hipError_t dispatch3(int v, const Args3& a, const void* src, void* dst, const ui
     case 4: return launch3<T, FORCE, 1024, 1, S>(a, src, dst, bits, words, s);
     case 5: return launch3<T, FORCE, 256, 2, S>(a, src, dst, bits, words, s);
     case 6: return launch3<T, FORCE, 64, 4, S>(a, src, dst, bits, words, s);
+    case 7: return launch3<T, FORCE, 256, 1, S | kXcdRemap>(a, src, dst, bits, words, s);
+    case 8: return launch3<T, FORCE, 512, 1, S | kXcdRemap>(a, src, dst, bits, words, s);
+    case 9: return launch3<T, FORCE, 128, 1, S | kXcdRemap>(a, src, dst, bits, words, s);
     default: return hipErrorInvalidValue;
     }
 }
