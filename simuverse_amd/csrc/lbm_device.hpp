@@ -162,6 +162,8 @@ struct PhysArgs {
     float fa[9];      // 3 fcoef w_i (e_i.g)
     float fb[9];      // 9 fcoef w_i (e_i.g)
     float fc[3];      // 3 fcoef w_k for the rest, axis and diagonal weights
+    float rom;        // 1 - omega
+    float ow[3];      // omega w_k
 };
 
 // Fills fcoef and the Guo coefficient tables from omega, gx, gy.
@@ -176,6 +178,10 @@ __host__ __device__ inline void set_force_terms(PhysArgs& a)
     a.fc[0] = 3.0f * a.fcoef * w_phys(0);
     a.fc[1] = 3.0f * a.fcoef * w_phys(1);
     a.fc[2] = 3.0f * a.fcoef * w_phys(5);
+    a.rom = 1.0f - a.omega;
+    a.ow[0] = a.omega * w_phys(0);
+    a.ow[1] = a.omega * w_phys(1);
+    a.ow[2] = a.omega * w_phys(5);
 }
 
 // Weight class of direction i: 0 rest, 1 axis, 2 diagonal.
@@ -204,9 +210,9 @@ struct BatchDomainDev {
 template <bool FORCE>
 __device__ inline void phys_moments(const PhysArgs& a, const float g[9], float& rho, float& ux, float& uy)
 {
-    float dr = 0.0f;
+    float dr = g[0];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) dr += g[i];
+    for (int i = 1; i < 9; ++i) dr += g[i];
     const float jx = (g[1] - g[3]) + (g[5] - g[6]) + (g[8] - g[7]);
     const float jy = (g[4] - g[2]) + (g[7] - g[5]) + (g[8] - g[6]);
     rho = 1.0f + dr;
@@ -220,30 +226,49 @@ __device__ inline void phys_moments(const PhysArgs& a, const float g[9], float& 
     }
 }
 
-// BGK (+ Guo forcing) in deviation form: g*_i = g_i - omega (g_i - geq_i) + F_i
-// with geq_i = feq_i - w_i = w_i (rho - 1) + w_i rho (e.u (3 + 4.5 e.u) - 1.5 u^2)
-// and F_i = fa_i + fb_i (e_i.u) - fc_k (u.g) (PhysArgs): nine fused
-// multiply-adds per direction, the per-cell products w_k rho, w_k (rho - 1)
-// and u.g shared by all directions of a weight class.
+// e_i.u for direction i (adds and negations only).
+__device__ __forceinline__ float e_dot_u(int i, float ux, float uy)
+{
+    const int ex = ex_of(i), ey = ey_of(i);
+    return ex == 0 ? (ey == 0 ? 0.0f : (ey > 0 ? uy : -uy))
+                   : (ey == 0 ? (ex > 0 ? ux : -ux)
+                              : (ex > 0 ? (ey > 0 ? ux + uy : ux - uy) : (ey > 0 ? uy - ux : -ux - uy)));
+}
+
+// BGK (+ Guo forcing) in deviation form:
+//   g*_i = (1 - omega) g_i + omega geq_i + F_i
+// with omega geq_i = omega w_k (rho - 1) + omega w_k rho q_i,
+// q_i = e.u (3 + 4.5 e.u) - 1.5 u^2, and F_i = fa_i + fb_i (e_i.u) - fc_k (u.g)
+// (PhysArgs).  Opposite directions share s = fb_i (e_i.u) - fc_k (u.g)
+// (fa and fb flip sign with e_i, and so does e_i.u), so F_i = s + fa_i and
+// F_inv(i) = s - fa_i.  About five fused multiply-adds per direction.
 template <bool FORCE>
 __device__ inline void phys_collide(const PhysArgs& a, float g[9], float rho, float ux, float uy)
 {
     const float dr = rho - 1.0f;
     const float usq = 1.5f * fmaf(ux, ux, uy * uy);
-    const float wr[3] = {w_phys(0) * rho, w_phys(1) * rho, w_phys(5) * rho};
-    const float wd[3] = {w_phys(0) * dr, w_phys(1) * dr, w_phys(5) * dr};
-    const float ug = FORCE ? fmaf(ux, a.gx, uy * a.gy) : 0.0f;
+    const float owr[3] = {a.ow[0] * rho, a.ow[1] * rho, a.ow[2] * rho};
+    const float owd[3] = {a.ow[0] * dr, a.ow[1] * dr, a.ow[2] * dr};
+    float F[9];
+    if (FORCE) {
+        const float ug = fmaf(ux, a.gx, uy * a.gy);
+        const float fu[3] = {-a.fc[0] * ug, -a.fc[1] * ug, -a.fc[2] * ug};
+        F[0] = fu[0];  // e_0 = 0: fa_0 = fb_0 = 0
+#pragma unroll
+        for (int i = 1; i < 9; ++i) {
+            if (inv_of(i) < i) continue;  // each pair once, from its lower index
+            const float sp = fmaf(a.fb[i], e_dot_u(i, ux, uy), fu[wclass(i)]);
+            F[i] = sp + a.fa[i];
+            F[inv_of(i)] = sp - a.fa[i];
+        }
+    }
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        const int ex = ex_of(i), ey = ey_of(i), k = wclass(i);
-        const float eu = ex == 0 ? (ey == 0 ? 0.0f : (ey > 0 ? uy : -uy))
-                                 : (ey == 0 ? (ex > 0 ? ux : -ux)
-                                            : (ex > 0 ? (ey > 0 ? ux + uy : ux - uy) : (ey > 0 ? uy - ux : -ux - uy)));
-        const float q = fmaf(eu, fmaf(4.5f, eu, 3.0f), -usq);
-        const float geq = fmaf(wr[k], q, wd[k]);
-        float out = fmaf(-a.omega, g[i] - geq, g[i]);
-        if (FORCE) out += fmaf(a.fb[i], eu, fmaf(-a.fc[k], ug, a.fa[i]));
-        g[i] = out;
+        const int k = wclass(i);
+        const float eu = e_dot_u(i, ux, uy);
+        const float q = i == 0 ? -usq : fmaf(eu, fmaf(4.5f, eu, 3.0f), -usq);
+        const float t = FORCE ? fmaf(g[i], a.rom, F[i]) : g[i] * a.rom;
+        g[i] = fmaf(owr[k], q, owd[k] + t);
     }
 }
 

@@ -545,8 +545,21 @@ __device__ inline void two_step_tile(const PhysArgs& a, const T* __restrict__ sr
     constexpr int kIter1 = (RC + BS - 1) / BS, kIter2 = (BW * BH) / BS;
     static_assert((BW * BH) % BS == 0, "tile must be a multiple of the block");
     const int nx = a.nx, ny = a.ny;
+    // Step-2 neighbourhoods first: their mask loads do not depend on step 1
+    // and complete while the block computes it.
+    CellNbr nb2[kIter2];
+#pragma unroll
+    for (int m = 0; m < kIter2; ++m) {
+        const int k = (int)threadIdx.x + BS * m;
+        const int x = x0 + k % BW, y = y0 + k / BW;
+        if (x < nx && y < ny) nb2[m] = cell_nbr(a, bits, words, x, y);
+        else nb2[m].s0 = 2u;  // outside the lattice: skipped like a solid cell
+    }
     // Step 1: region cells (q, r) = (x0 - 1 + q, y0 - 1 + r), y wrapped, x
-    // wrapped when periodic.
+    // wrapped when periodic.  With periodic x, solid cells are computed too
+    // (from whatever their slots hold) instead of testing the mask: step 2
+    // never reads their LDS entries, and the gather starts without waiting
+    // for a mask load.
 #pragma unroll
     for (int m = 0; m < kIter1; ++m) {
         const int k = (int)threadIdx.x + BS * m;
@@ -557,8 +570,15 @@ __device__ inline void two_step_tile(const PhysArgs& a, const T* __restrict__ sr
         if (XEDGE == 1 && (x < 0 || x >= nx)) continue;  // outside the channel: never read
         x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
         y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
-        const CellNbr nb = cell_nbr(a, bits, words, x, y);
-        if (nb.s0 & 2u) continue;  // solid: never read by step 2
+        CellNbr nb;
+        nb.ym = (y == 0) ? ny - 1 : y - 1;
+        nb.yp = (y == ny - 1) ? 0 : y + 1;
+        nb.sm = nb.sp = 0u;
+        nb.s0 = 0u;
+        if (XEDGE == 1) {
+            nb.s0 = solid3(bits, words, y, x);  // the outlet gathers through its west bit
+            if (nb.s0 & 2u) continue;
+        }
         float g[9];
         cell_gather<T, XEDGE, FLAGS>(a, src, nb, x, y, g);
         float rho, ux, uy;
@@ -574,37 +594,45 @@ __device__ inline void two_step_tile(const PhysArgs& a, const T* __restrict__ sr
         const int k = (int)threadIdx.x + BS * m;
         const int row = k / BW, col = k % BW;
         const int x = x0 + col, y = y0 + row;
-        if (x >= nx || y >= ny) continue;
-        const CellNbr nb = cell_nbr(a, bits, words, x, y);
+        const CellNbr& nb = nb2[m];
         if (nb.s0 & 2u) continue;
-        // gathering cell: the outlet column gathers as its west neighbour
-        int cx = x;
-        CellNbr gb = nb;
-        if (XEDGE == 1 && x == nx - 1 && !(nb.s0 & 1u)) {
-            cx = x - 1;
-            gb = cell_nbr(a, bits, words, cx, y);
-        }
-        const int c = (row + 1) * RW + (cx - x0 + 1);
         float g[9];
+        const bool outlet = XEDGE == 1 && x == nx - 1 && !(nb.s0 & 1u);
+        if ((nb.sm | nb.sp | (nb.s0 & 5u)) == 0u && !outlet && (XEDGE == 0 || x > 0)) {
+            // no solid neighbour, no edge: plain pulls from the region
+            const int c = (row + 1) * RW + (col + 1);
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            const int ex = ex_of(i), ey = ey_of(i);
-            // source cell cx - e_i: bit (1 - ex) of row y - ey
-            const uint32_t srow = ey == 1 ? gb.sm : (ey == -1 ? gb.sp : gb.s0);
-            bool own = (srow >> (1 - ex)) & 1u;
-            if (XEDGE == 1 && ex != 0) own = own || cx - ex < 0 || cx - ex >= nx;
-            const T v = own ? lds[inv_of(i)][c] : lds[i][c - ey * RW - ex];
-            g[i] = Store<T>::from_storage(v, a.inv_scale);
-        }
-        if (XEDGE == 1 && x == 0) {
-            const float u0 = a.inlet_ux;
-            const float G = g[0] + g[2] + g[4] + 2.0f * (g[3] + g[6] + g[7]);
-            const float rho = (1.0f + G) / (1.0f - u0);
-            const float ru = rho * u0;
-            const float h = 0.5f * (g[4] - g[2]);
-            g[1] = g[3] + (2.0f / 3.0f) * ru;
-            g[5] = g[7] + h + (1.0f / 6.0f) * ru;
-            g[8] = g[6] - h + (1.0f / 6.0f) * ru;
+            for (int i = 0; i < 9; ++i)
+                g[i] = Store<T>::from_storage(lds[i][c - ey_of(i) * RW - ex_of(i)], a.inv_scale);
+        } else {
+            // gathering cell: the outlet column gathers as its west neighbour
+            int cx = x;
+            CellNbr gb = nb;
+            if (outlet) {
+                cx = x - 1;
+                gb = cell_nbr(a, bits, words, cx, y);
+            }
+            const int c = (row + 1) * RW + (cx - x0 + 1);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                const int ex = ex_of(i), ey = ey_of(i);
+                // source cell cx - e_i: bit (1 - ex) of row y - ey
+                const uint32_t srow = ey == 1 ? gb.sm : (ey == -1 ? gb.sp : gb.s0);
+                bool own = (srow >> (1 - ex)) & 1u;
+                if (XEDGE == 1 && ex != 0) own = own || cx - ex < 0 || cx - ex >= nx;
+                const T v = own ? lds[inv_of(i)][c] : lds[i][c - ey * RW - ex];
+                g[i] = Store<T>::from_storage(v, a.inv_scale);
+            }
+            if (XEDGE == 1 && x == 0) {
+                const float u0 = a.inlet_ux;
+                const float G = g[0] + g[2] + g[4] + 2.0f * (g[3] + g[6] + g[7]);
+                const float rho = (1.0f + G) / (1.0f - u0);
+                const float ru = rho * u0;
+                const float h = 0.5f * (g[4] - g[2]);
+                g[1] = g[3] + (2.0f / 3.0f) * ru;
+                g[5] = g[7] + h + (1.0f / 6.0f) * ru;
+                g[8] = g[6] - h + (1.0f / 6.0f) * ru;
+            }
         }
         float rho, ux, uy;
         phys_moments<FORCE>(a, g, rho, ux, uy);
