@@ -139,10 +139,11 @@ __device__ __forceinline__ void cell_store(const PhysArgs& a, T* __restrict__ ds
 {
     const int nx = a.nx;
     const uint32_t r0 = (uint32_t)y * a.rs;
-    const uint32_t pc = r0 + (uint32_t)x;
+    // one per-lane pointer; the direction offsets are uniform across the wave
+    T* const own = dst + (r0 + (uint32_t)x);
 #pragma unroll
     for (int i = 0; i < 9; ++i)
-        st<(FLAGS & kNtStore) != 0>(dst + pc + (uint32_t)i * a.plane, Store<T>::to_storage(g[i], a.scale));
+        st<(FLAGS & kNtStore) != 0>(own + (size_t)((uint32_t)i * a.plane), Store<T>::to_storage(g[i], a.scale));
     if ((nb.sm | nb.sp | (nb.s0 & 5u)) == 0u) return;
     int xm = x - 1, xp = x + 1;
     if (XEDGE == 0) {
@@ -545,23 +546,59 @@ __device__ inline void two_step_tile(const PhysArgs& a, const T* __restrict__ sr
     constexpr int kIter1 = (RC + BS - 1) / BS, kIter2 = (BW * BH) / BS;
     static_assert((BW * BH) % BS == 0, "tile must be a multiple of the block");
     const int nx = a.nx, ny = a.ny;
-    // Step-2 neighbourhoods first: their mask loads do not depend on step 1
-    // and complete while the block computes it.
-    CellNbr nb2[kIter2];
-#pragma unroll
-    for (int m = 0; m < kIter2; ++m) {
+    // Step-2 neighbourhoods first when they cost few registers (binary32,
+    // whose occupancy LDS bounds anyway): their mask loads do not depend on
+    // step 1 and complete while the block computes it.
+    constexpr bool kPrefetch = sizeof(T) == 4 && kIter2 <= 2;
+    auto nbr2 = [&](int m) {
         const int k = (int)threadIdx.x + BS * m;
         const int x = x0 + k % BW, y = y0 + k / BW;
-        if (x < nx && y < ny) nb2[m] = cell_nbr(a, bits, words, x, y);
-        else nb2[m].s0 = 2u;  // outside the lattice: skipped like a solid cell
+        CellNbr nb;
+        if (x < nx && y < ny) nb = cell_nbr(a, bits, words, x, y);
+        else nb.s0 = 2u;  // outside the lattice: skipped like a solid cell
+        return nb;
+    };
+    CellNbr nb2[kPrefetch ? kIter2 : 1];
+    if constexpr (kPrefetch) {
+#pragma unroll
+        for (int m = 0; m < kIter2; ++m) nb2[m] = nbr2(m);
     }
     // Step 1: region cells (q, r) = (x0 - 1 + q, y0 - 1 + r), y wrapped, x
     // wrapped when periodic.  With periodic x, solid cells are computed too
     // (from whatever their slots hold) instead of testing the mask: step 2
     // never reads their LDS entries, and the gather starts without waiting
     // for a mask load.
+    // A tile whose region and its gather neighbourhood lie inside the lattice
+    // (no wrap, no inlet or outlet column) takes the same pulls with one
+    // pointer per cell and wave-uniform direction offsets (binary32; in
+    // binary16 the extra path costs more occupancy than it saves).
+    const bool inner = sizeof(T) == 4 && x0 >= 2 && x0 + BW + 1 <= nx - 1 && y0 >= 2 && y0 + BH + 1 <= ny - 1;
+    if (inner) {
 #pragma unroll
-    for (int m = 0; m < kIter1; ++m) {
+        for (int m = 0; m < kIter1; ++m) {
+            const int k = (int)threadIdx.x + BS * m;
+            if (k >= RC) break;
+            const int r = k / RW, q = k % RW;
+            const int x = x0 - 1 + q, y = y0 - 1 + r;
+            if (XEDGE == 1 && (solid3(bits, words, y, x) & 2u)) continue;
+            const T* const cell = src + ((uint32_t)y * a.rs + (uint32_t)x);
+            float g[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                // x - e_i: row y - ey_i, column x - ex_i
+                const ptrdiff_t off = (ptrdiff_t)((uint32_t)i * a.plane) - (ptrdiff_t)ey_of(i) * (ptrdiff_t)a.rs -
+                                      (ptrdiff_t)ex_of(i);
+                g[i] = Store<T>::from_storage(ld<(FLAGS & kNtLoad) != 0>(cell + off), a.inv_scale);
+            }
+            float rho, ux, uy;
+            phys_moments<FORCE>(a, g, rho, ux, uy);
+            phys_collide<FORCE>(a, g, rho, ux, uy);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) lds[i][k] = Store<T>::to_storage(g[i], a.scale);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < kIter1 && !inner; ++m) {
         const int k = (int)threadIdx.x + BS * m;
         if (k >= RC) break;
         const int r = k / RW, q = k % RW;
@@ -594,7 +631,9 @@ __device__ inline void two_step_tile(const PhysArgs& a, const T* __restrict__ sr
         const int k = (int)threadIdx.x + BS * m;
         const int row = k / BW, col = k % BW;
         const int x = x0 + col, y = y0 + row;
-        const CellNbr& nb = nb2[m];
+        CellNbr nb;
+        if constexpr (kPrefetch) nb = nb2[m];
+        else nb = nbr2(m);
         if (nb.s0 & 2u) continue;
         float g[9];
         const bool outlet = XEDGE == 1 && x == nx - 1 && !(nb.s0 & 1u);
