@@ -133,7 +133,7 @@ def run_batch(args):
     if args.variant >= 0:
         b.set_variant(args.variant)
     else:
-        v, ms = b.autotune(steps=40)
+        v, ms = b.autotune(steps=100)
         log(f"batch autotune: variant {v} {ms:.3f} ms/step")
     b.run(args.warmup)
     b.synchronize()
