@@ -141,6 +141,53 @@ struct Store<_Float16> {
     __device__ static inline float scale_out(float g, float scale) { return g * scale; }
 };
 
+// Conversions of a kernel that works in the scaled binary16 domain (SC) or
+// in plain deviations: SC keeps g * scale in registers, so loads and
+// stores only convert the format.  The value is pinned in a register
+// before the narrowing conversion (an empty asm barrier): otherwise the
+// compiler may fold the collision's last fused multiply-add into the
+// conversion (v_fma_mixlo_f16, one rounding), while the unscaled form
+// rounds twice (to binary32, then to binary16) and the kernels would no
+// longer agree bit for bit.
+__device__ __forceinline__ float pin_f32(float v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+template <typename T, bool SC>
+struct Dom {
+    __device__ static inline float in(T v, float inv_scale)
+    {
+        if constexpr (SC) return (float)v;
+        else return Store<T>::from_storage(v, inv_scale);
+    }
+    __device__ static inline T out(float g, float scale)
+    {
+        if constexpr (SC) return (T)pin_f32(g);
+        else return Store<T>::to_storage(g, scale);
+    }
+    // element already widened to binary32
+    __device__ static inline float in_w(float v, float inv_scale)
+    {
+        if constexpr (SC) return v;
+        else return Store<T>::scale_in(v, inv_scale);
+    }
+    __device__ static inline float out_w(float g, float scale)
+    {
+        if constexpr (SC) return pin_f32(g);
+        else return Store<T>::scale_out(g, scale);
+    }
+    __device__ static inline void push(T* p, uint32_t k, float g, float w, float scale)
+    {
+        if constexpr (SC) p[k] = (T)pin_f32(g);
+        else Store<T>::store_dev(p, k, g, w, scale);
+    }
+};
+// Whether a hot kernel keeps storage type T in the scaled domain.
+template <typename T>
+constexpr bool kScaledDom = Store<T>::bytes == 2;
+
 // Arguments shared by the physics-mode kernels of one domain.  Population i
 // of cell (x, y) is element y*rs + x + i*plane of a buffer:
 //   row-interleaved  plane = pitch, rs = 9*pitch (the nine rows of one y are
@@ -164,6 +211,11 @@ struct PhysArgs {
     float fc[3];      // 3 fcoef w_k for the rest, axis and diagonal weights
     float rom;        // 1 - omega
     float ow[3];      // omega w_k
+    // The same constants times `scale`, for kernels that keep binary16
+    // deviations in their scaled form g * 2^shift through the step: every
+    // product and sum of the collision is then the unscaled one times a
+    // power of two, rounded identically, so both forms give the same bits.
+    float sfa[9], sfb[9], sfc[3], sow[3];
 };
 
 // Fills fcoef and the Guo coefficient tables from omega, gx, gy.
@@ -182,6 +234,14 @@ __host__ __device__ inline void set_force_terms(PhysArgs& a)
     a.ow[0] = a.omega * w_phys(0);
     a.ow[1] = a.omega * w_phys(1);
     a.ow[2] = a.omega * w_phys(5);
+    for (int i = 0; i < 9; ++i) {
+        a.sfa[i] = a.fa[i] * a.scale;
+        a.sfb[i] = a.fb[i] * a.scale;
+    }
+    for (int k = 0; k < 3; ++k) {
+        a.sfc[k] = a.fc[k] * a.scale;
+        a.sow[k] = a.ow[k] * a.scale;
+    }
 }
 
 // Weight class of direction i: 0 rest, 1 axis, 2 diagonal.
@@ -207,14 +267,20 @@ struct BatchDomainDev {
 };
 
 // Moments of the post-streaming deviations (Guo velocity with FORCE).
-template <bool FORCE>
+// SC: g holds scaled deviations (g * scale; the sums are rescaled exactly).
+template <bool FORCE, bool SC = false>
 __device__ inline void phys_moments(const PhysArgs& a, const float g[9], float& rho, float& ux, float& uy)
 {
     float dr = g[0];
 #pragma unroll
     for (int i = 1; i < 9; ++i) dr += g[i];
-    const float jx = (g[1] - g[3]) + (g[5] - g[6]) + (g[8] - g[7]);
-    const float jy = (g[4] - g[2]) + (g[7] - g[5]) + (g[8] - g[6]);
+    float jx = (g[1] - g[3]) + (g[5] - g[6]) + (g[8] - g[7]);
+    float jy = (g[4] - g[2]) + (g[7] - g[5]) + (g[8] - g[6]);
+    if (SC) {
+        dr *= a.inv_scale;
+        jx *= a.inv_scale;
+        jy *= a.inv_scale;
+    }
     rho = 1.0f + dr;
     const float ir = __builtin_amdgcn_rcpf(rho);  // hardware reciprocal, 1 ulp
     if (FORCE) {
@@ -242,24 +308,30 @@ __device__ __forceinline__ float e_dot_u(int i, float ux, float uy)
 // (PhysArgs).  Opposite directions share s = fb_i (e_i.u) - fc_k (u.g)
 // (fa and fb flip sign with e_i, and so does e_i.u), so F_i = s + fa_i and
 // F_inv(i) = s - fa_i.  About five fused multiply-adds per direction.
-template <bool FORCE>
+// SC: g holds scaled deviations and the result is scaled (the constants
+// sow, sfa, sfb, sfc carry the scale).
+template <bool FORCE, bool SC = false>
 __device__ inline void phys_collide(const PhysArgs& a, float g[9], float rho, float ux, float uy)
 {
+    const float* const ow = SC ? a.sow : a.ow;
+    const float* const fa = SC ? a.sfa : a.fa;
+    const float* const fb = SC ? a.sfb : a.fb;
+    const float* const fc = SC ? a.sfc : a.fc;
     const float dr = rho - 1.0f;
     const float usq = 1.5f * fmaf(ux, ux, uy * uy);
-    const float owr[3] = {a.ow[0] * rho, a.ow[1] * rho, a.ow[2] * rho};
-    const float owd[3] = {a.ow[0] * dr, a.ow[1] * dr, a.ow[2] * dr};
+    const float owr[3] = {ow[0] * rho, ow[1] * rho, ow[2] * rho};
+    const float owd[3] = {ow[0] * dr, ow[1] * dr, ow[2] * dr};
     float F[9];
     if (FORCE) {
         const float ug = fmaf(ux, a.gx, uy * a.gy);
-        const float fu[3] = {-a.fc[0] * ug, -a.fc[1] * ug, -a.fc[2] * ug};
+        const float fu[3] = {-fc[0] * ug, -fc[1] * ug, -fc[2] * ug};
         F[0] = fu[0];  // e_0 = 0: fa_0 = fb_0 = 0
 #pragma unroll
         for (int i = 1; i < 9; ++i) {
             if (inv_of(i) < i) continue;  // each pair once, from its lower index
-            const float sp = fmaf(a.fb[i], e_dot_u(i, ux, uy), fu[wclass(i)]);
-            F[i] = sp + a.fa[i];
-            F[inv_of(i)] = sp - a.fa[i];
+            const float sp = fmaf(fb[i], e_dot_u(i, ux, uy), fu[wclass(i)]);
+            F[i] = sp + fa[i];
+            F[inv_of(i)] = sp - fa[i];
         }
     }
 #pragma unroll

@@ -91,9 +91,31 @@ __device__ __forceinline__ CellNbr cell_nbr(const PhysArgs& a, const uint32_t* _
     return nb;
 }
 
+// Zou & He velocity inlet on the gathered deviations of a column-0 cell
+// (w0+w2+w4+2(w3+w6+w7) = 1); SC: g is scaled, and the unscaled sums and
+// terms are formed exactly (power-of-two factors) so both forms agree.
+template <bool SC>
+__device__ __forceinline__ void zou_he_inlet(const PhysArgs& a, float g[9])
+{
+    const float u0 = a.inlet_ux;
+    float G = g[0] + g[2] + g[4] + 2.0f * (g[3] + g[6] + g[7]);
+    if (SC) G *= a.inv_scale;
+    const float rho = (1.0f + G) / (1.0f - u0);
+    const float ru = rho * u0;
+    const float h = 0.5f * (g[4] - g[2]);
+    float c23 = (2.0f / 3.0f) * ru, c16 = (1.0f / 6.0f) * ru;
+    if (SC) {
+        c23 *= a.scale;
+        c16 *= a.scale;
+    }
+    g[1] = g[3] + c23;
+    g[5] = g[7] + h + c16;
+    g[8] = g[6] - h + c16;
+}
+
 // Post-streaming deviations of the fluid cell (x, y) (see phys_gather; the
 // outlet test uses the west bit of the mask).
-template <typename T, int XEDGE, int FLAGS>
+template <typename T, int XEDGE, int FLAGS, bool SC = false>
 __device__ __forceinline__ void cell_gather(const PhysArgs& a, const T* __restrict__ src, const CellNbr& nb, int x,
                                             int y, float g[9])
 {
@@ -115,25 +137,16 @@ __device__ __forceinline__ void cell_gather(const PhysArgs& a, const T* __restri
         bool oob = false;
         if (XEDGE == 1 && ex != 0) oob = (sx < 0) || (sx >= nx);
         const uint32_t k = oob ? cc + (uint32_t)inv_of(i) * a.plane : srow + (uint32_t)sx + (uint32_t)i * a.plane;
-        g[i] = Store<T>::from_storage(ld<(FLAGS & kNtLoad) != 0>(src + k), a.inv_scale);
+        g[i] = Dom<T, SC>::in(ld<(FLAGS & kNtLoad) != 0>(src + k), a.inv_scale);
     }
-    if (XEDGE == 1 && x == 0) {
-        const float u0 = a.inlet_ux;
-        const float G = g[0] + g[2] + g[4] + 2.0f * (g[3] + g[6] + g[7]);
-        const float rho = (1.0f + G) / (1.0f - u0);
-        const float ru = rho * u0;
-        const float h = 0.5f * (g[4] - g[2]);
-        g[1] = g[3] + (2.0f / 3.0f) * ru;
-        g[5] = g[7] + h + (1.0f / 6.0f) * ru;
-        g[8] = g[6] - h + (1.0f / 6.0f) * ru;
-    }
+    if (XEDGE == 1 && x == 0) zou_he_inlet<SC>(a, g);
 }
 
 // Stores the post-collision deviations of the fluid cell (x, y) in its own
 // slots, and pushes f_i into slot inv(i) of each solid neighbour x + e_i
 // (the mask's edge bits are empty for a non-periodic x, so nothing is pushed
 // outside).
-template <typename T, int XEDGE, int FLAGS>
+template <typename T, int XEDGE, int FLAGS, bool SC = false>
 __device__ __forceinline__ void cell_store(const PhysArgs& a, T* __restrict__ dst, const CellNbr& nb, int x, int y,
                                            const float g[9])
 {
@@ -143,7 +156,7 @@ __device__ __forceinline__ void cell_store(const PhysArgs& a, T* __restrict__ ds
     T* const own = dst + (r0 + (uint32_t)x);
 #pragma unroll
     for (int i = 0; i < 9; ++i)
-        st<(FLAGS & kNtStore) != 0>(own + (size_t)((uint32_t)i * a.plane), Store<T>::to_storage(g[i], a.scale));
+        st<(FLAGS & kNtStore) != 0>(own + (size_t)((uint32_t)i * a.plane), Dom<T, SC>::out(g[i], a.scale));
     if ((nb.sm | nb.sp | (nb.s0 & 5u)) == 0u) return;
     int xm = x - 1, xp = x + 1;
     if (XEDGE == 0) {
@@ -158,7 +171,7 @@ __device__ __forceinline__ void cell_store(const PhysArgs& a, T* __restrict__ ds
         if (!((srow >> (1 + ex)) & 1u)) continue;
         const int tx = ex == 1 ? xp : (ex == -1 ? xm : x);
         const uint32_t trow = ey == 1 ? rp : (ey == -1 ? rm : r0);
-        Store<T>::store_dev(dst, trow + (uint32_t)tx + (uint32_t)inv_of(i) * a.plane, g[i], w_phys(i), a.scale);
+        Dom<T, SC>::push(dst, trow + (uint32_t)tx + (uint32_t)inv_of(i) * a.plane, g[i], w_phys(i), a.scale);
     }
 }
 
@@ -182,9 +195,11 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
         return;
     }
     float g[9];
-    cell_gather<T, XEDGE, FLAGS>(a, src, nb, x, y, g);
+    // the learned correction works on unscaled deviations
+    constexpr bool SC = kScaledDom<T> && !LEARNED;
+    cell_gather<T, XEDGE, FLAGS, SC>(a, src, nb, x, y, g);
     float rho, ux, uy;
-    phys_moments<FORCE>(a, g, rho, ux, uy);
+    phys_moments<FORCE, SC>(a, g, rho, ux, uy);
     if (EXPORT) {
         macro[c] = ux;
         macro[c + n] = uy;
@@ -204,12 +219,12 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
             learned_correction(xin, y9, s_in, s_out, a.omega, corr);
         }
     }
-    phys_collide<FORCE>(a, g, rho, ux, uy);
+    phys_collide<FORCE, SC>(a, g, rho, ux, uy);
     if (learned) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) g[i] += corr[i];
     }
-    cell_store<T, XEDGE, FLAGS>(a, dst, nb, x, y, g);
+    cell_store<T, XEDGE, FLAGS, SC>(a, dst, nb, x, y, g);
 }
 
 // One step of the whole lattice.  Block = BS threads on a row segment of
@@ -298,6 +313,7 @@ __device__ inline void phys_cells4(const PhysArgs& a, const T* __restrict__ src,
     const uint32_t s0 = solid6(bits, words, y, x0);
     const uint32_t sp = solid6(bits, words, yp, x0);
     const uint32_t self = (s0 >> 1) & 15u;  // bit k: cell x0 + k is solid
+    constexpr bool SC = kScaledDom<T>;
     const uint32_t c0 = (uint32_t)y * (uint32_t)nx + (uint32_t)x0;  // cell (macro planes)
     const uint32_t p0 = r0 + (uint32_t)x0;                         // population element
     if (self == 15u) {
@@ -319,9 +335,9 @@ __device__ inline void phys_cells4(const PhysArgs& a, const T* __restrict__ src,
     for (int k = 0; k < 4; ++k) {
         float gc[9];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) gc[i] = Store<T>::scale_in(g[i][k], a.inv_scale);
+        for (int i = 0; i < 9; ++i) gc[i] = Dom<T, SC>::in_w(g[i][k], a.inv_scale);
         float rho, ux, uy;
-        phys_moments<FORCE>(a, gc, rho, ux, uy);
+        phys_moments<FORCE, SC>(a, gc, rho, ux, uy);
         if (EXPORT) {
             const bool s = (self >> k) & 1u;
             macro[c0 + k] = s ? 0.0f : ux;
@@ -329,7 +345,7 @@ __device__ inline void phys_cells4(const PhysArgs& a, const T* __restrict__ src,
             macro[c0 + k + 2 * n] = s ? 0.0f : rho;
             macro[c0 + k + 3 * n] = s ? 0.0f : 1.0f;
         }
-        phys_collide<FORCE>(a, gc, rho, ux, uy);
+        phys_collide<FORCE, SC>(a, gc, rho, ux, uy);
 #pragma unroll
         for (int i = 0; i < 9; ++i) g[i][k] = gc[i];
     }
@@ -338,7 +354,7 @@ __device__ inline void phys_cells4(const PhysArgs& a, const T* __restrict__ src,
         for (int i = 0; i < 9; ++i) {
             float o[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) o[k] = Store<T>::scale_out(g[i][k], a.scale);
+            for (int k = 0; k < 4; ++k) o[k] = Dom<T, SC>::out_w(g[i][k], a.scale);
             Vec4<T>::store(dst + p0 + (uint32_t)i * a.plane, o);
         }
     } else {
@@ -349,7 +365,7 @@ __device__ inline void phys_cells4(const PhysArgs& a, const T* __restrict__ src,
             if ((self >> k) & 1u) continue;
 #pragma unroll
             for (int i = 0; i < 9; ++i)
-                Store<T>::store_dev(dst, p0 + (uint32_t)k + (uint32_t)i * a.plane, g[i][k], w_phys(i), a.scale);
+                Dom<T, SC>::push(dst, p0 + (uint32_t)k + (uint32_t)i * a.plane, g[i][k], w_phys(i), a.scale);
         }
     }
     if ((sm | sp | s0) == 0u) return;
@@ -362,8 +378,8 @@ __device__ inline void phys_cells4(const PhysArgs& a, const T* __restrict__ src,
             const uint32_t srow = ey == 1 ? sp : (ey == -1 ? sm : s0);
             if (!((srow >> (k + 1 + ex)) & 1u)) continue;
             const uint32_t trow = ey == 1 ? rp : (ey == -1 ? rm : r0);
-            Store<T>::store_dev(dst, trow + (uint32_t)(x0 + k + ex) + (uint32_t)inv_of(i) * a.plane, g[i][k],
-                                w_phys(i), a.scale);
+            Dom<T, SC>::push(dst, trow + (uint32_t)(x0 + k + ex) + (uint32_t)inv_of(i) * a.plane, g[i][k],
+                             w_phys(i), a.scale);
         }
     }
 }
@@ -401,6 +417,7 @@ __device__ inline void phys_cells_shfl(const PhysArgs& a, const T* __restrict__ 
                                        float* __restrict__ macro, uint32_t n, int wbase, int lane, int y)
 {
     typedef T vecT __attribute__((ext_vector_type(V)));
+    constexpr bool SC = kScaledDom<T>;
     const int nx = a.nx, ny = a.ny;
     const int x0 = wbase + lane * V;
     const int ym = (y == 0) ? ny - 1 : y - 1;
@@ -418,19 +435,19 @@ __device__ inline void phys_cells_shfl(const PhysArgs& a, const T* __restrict__ 
         else v = *reinterpret_cast<const vecT*>(row + x0);
         float f[V];
 #pragma unroll
-        for (int k = 0; k < V; ++k) f[k] = Store<T>::from_storage(v[k], a.inv_scale);
+        for (int k = 0; k < V; ++k) f[k] = Dom<T, SC>::in(v[k], a.inv_scale);
         if (ex == 0) {
 #pragma unroll
             for (int k = 0; k < V; ++k) g[i][k] = f[k];
         } else if (ex == 1) {  // pulls from x - 1
             float prev = __shfl_up(f[V - 1], 1, 64);
-            const float edge = Store<T>::from_storage(row[xl], a.inv_scale);
+            const float edge = Dom<T, SC>::in(row[xl], a.inv_scale);
             g[i][0] = lane == 0 ? edge : prev;
 #pragma unroll
             for (int k = 1; k < V; ++k) g[i][k] = f[k - 1];
         } else {  // pulls from x + 1
             float next = __shfl_down(f[0], 1, 64);
-            const float edge = Store<T>::from_storage(row[xr], a.inv_scale);
+            const float edge = Dom<T, SC>::in(row[xr], a.inv_scale);
 #pragma unroll
             for (int k = 0; k < V - 1; ++k) g[i][k] = f[k + 1];
             g[i][V - 1] = lane == 63 ? edge : next;
@@ -453,7 +470,7 @@ __device__ inline void phys_cells_shfl(const PhysArgs& a, const T* __restrict__ 
 #pragma unroll
         for (int i = 0; i < 9; ++i) gc[i] = g[i][k];
         float rho, ux, uy;
-        phys_moments<FORCE>(a, gc, rho, ux, uy);
+        phys_moments<FORCE, SC>(a, gc, rho, ux, uy);
         if (EXPORT) {
             const bool sk = (self >> k) & 1u;
             macro[c0 + k] = sk ? 0.0f : ux;
@@ -461,7 +478,7 @@ __device__ inline void phys_cells_shfl(const PhysArgs& a, const T* __restrict__ 
             macro[c0 + k + 2 * n] = sk ? 0.0f : rho;
             macro[c0 + k + 3 * n] = sk ? 0.0f : 1.0f;
         }
-        phys_collide<FORCE>(a, gc, rho, ux, uy);
+        phys_collide<FORCE, SC>(a, gc, rho, ux, uy);
 #pragma unroll
         for (int i = 0; i < 9; ++i) g[i][k] = gc[i];
     }
@@ -470,7 +487,7 @@ __device__ inline void phys_cells_shfl(const PhysArgs& a, const T* __restrict__ 
         for (int i = 0; i < 9; ++i) {
             vecT o;
 #pragma unroll
-            for (int k = 0; k < V; ++k) o[k] = Store<T>::to_storage(g[i][k], a.scale);
+            for (int k = 0; k < V; ++k) o[k] = Dom<T, SC>::out(g[i][k], a.scale);
             vecT* q = reinterpret_cast<vecT*>(dst + p0 + (uint32_t)i * a.plane);
             if constexpr ((FLAGS & kNtStore) != 0) __builtin_nontemporal_store(o, q);
             else *q = o;
@@ -483,7 +500,7 @@ __device__ inline void phys_cells_shfl(const PhysArgs& a, const T* __restrict__ 
             if ((self >> k) & 1u) continue;
 #pragma unroll
             for (int i = 0; i < 9; ++i)
-                Store<T>::store_dev(dst, p0 + (uint32_t)k + (uint32_t)i * a.plane, g[i][k], w_phys(i), a.scale);
+                Dom<T, SC>::push(dst, p0 + (uint32_t)k + (uint32_t)i * a.plane, g[i][k], w_phys(i), a.scale);
         }
     }
     if ((sm | sp | s0) == 0u) return;
@@ -498,7 +515,7 @@ __device__ inline void phys_cells_shfl(const PhysArgs& a, const T* __restrict__ 
             int tx = x0 + k + ex;
             tx = tx < 0 ? nx - 1 : (tx >= nx ? 0 : tx);
             const uint32_t trow = ey == 1 ? rp : (ey == -1 ? rm : r0);
-            Store<T>::store_dev(dst, trow + (uint32_t)tx + (uint32_t)inv_of(i) * a.plane, g[i][k], w_phys(i),
+            Dom<T, SC>::push(dst, trow + (uint32_t)tx + (uint32_t)inv_of(i) * a.plane, g[i][k], w_phys(i),
                                 a.scale);
         }
     }
@@ -546,6 +563,7 @@ __device__ inline void two_step_tile(const PhysArgs& a, const T* __restrict__ sr
     constexpr int kIter1 = (RC + BS - 1) / BS, kIter2 = (BW * BH) / BS;
     static_assert((BW * BH) % BS == 0, "tile must be a multiple of the block");
     const int nx = a.nx, ny = a.ny;
+    constexpr bool SC = kScaledDom<T>;
     // Step-2 neighbourhoods first when they cost few registers (binary32,
     // whose occupancy LDS bounds anyway): their mask loads do not depend on
     // step 1 and complete while the block computes it.
@@ -588,13 +606,13 @@ __device__ inline void two_step_tile(const PhysArgs& a, const T* __restrict__ sr
                 // x - e_i: row y - ey_i, column x - ex_i
                 const ptrdiff_t off = (ptrdiff_t)((uint32_t)i * a.plane) - (ptrdiff_t)ey_of(i) * (ptrdiff_t)a.rs -
                                       (ptrdiff_t)ex_of(i);
-                g[i] = Store<T>::from_storage(ld<(FLAGS & kNtLoad) != 0>(cell + off), a.inv_scale);
+                g[i] = Dom<T, SC>::in(ld<(FLAGS & kNtLoad) != 0>(cell + off), a.inv_scale);
             }
             float rho, ux, uy;
-            phys_moments<FORCE>(a, g, rho, ux, uy);
-            phys_collide<FORCE>(a, g, rho, ux, uy);
+            phys_moments<FORCE, SC>(a, g, rho, ux, uy);
+            phys_collide<FORCE, SC>(a, g, rho, ux, uy);
 #pragma unroll
-            for (int i = 0; i < 9; ++i) lds[i][k] = Store<T>::to_storage(g[i], a.scale);
+            for (int i = 0; i < 9; ++i) lds[i][k] = Dom<T, SC>::out(g[i], a.scale);
         }
     }
 #pragma unroll
@@ -617,12 +635,12 @@ __device__ inline void two_step_tile(const PhysArgs& a, const T* __restrict__ sr
             if (nb.s0 & 2u) continue;
         }
         float g[9];
-        cell_gather<T, XEDGE, FLAGS>(a, src, nb, x, y, g);
+        cell_gather<T, XEDGE, FLAGS, SC>(a, src, nb, x, y, g);
         float rho, ux, uy;
-        phys_moments<FORCE>(a, g, rho, ux, uy);
-        phys_collide<FORCE>(a, g, rho, ux, uy);
+        phys_moments<FORCE, SC>(a, g, rho, ux, uy);
+        phys_collide<FORCE, SC>(a, g, rho, ux, uy);
 #pragma unroll
-        for (int i = 0; i < 9; ++i) lds[i][k] = Store<T>::to_storage(g[i], a.scale);
+        for (int i = 0; i < 9; ++i) lds[i][k] = Dom<T, SC>::out(g[i], a.scale);
     }
     __syncthreads();
     // Step 2: tile cells, gathered from LDS.
@@ -642,7 +660,7 @@ __device__ inline void two_step_tile(const PhysArgs& a, const T* __restrict__ sr
             const int c = (row + 1) * RW + (col + 1);
 #pragma unroll
             for (int i = 0; i < 9; ++i)
-                g[i] = Store<T>::from_storage(lds[i][c - ey_of(i) * RW - ex_of(i)], a.inv_scale);
+                g[i] = Dom<T, SC>::in(lds[i][c - ey_of(i) * RW - ex_of(i)], a.inv_scale);
         } else {
             // gathering cell: the outlet column gathers as its west neighbour
             int cx = x;
@@ -660,23 +678,14 @@ __device__ inline void two_step_tile(const PhysArgs& a, const T* __restrict__ sr
                 bool own = (srow >> (1 - ex)) & 1u;
                 if (XEDGE == 1 && ex != 0) own = own || cx - ex < 0 || cx - ex >= nx;
                 const T v = own ? lds[inv_of(i)][c] : lds[i][c - ey * RW - ex];
-                g[i] = Store<T>::from_storage(v, a.inv_scale);
+                g[i] = Dom<T, SC>::in(v, a.inv_scale);
             }
-            if (XEDGE == 1 && x == 0) {
-                const float u0 = a.inlet_ux;
-                const float G = g[0] + g[2] + g[4] + 2.0f * (g[3] + g[6] + g[7]);
-                const float rho = (1.0f + G) / (1.0f - u0);
-                const float ru = rho * u0;
-                const float h = 0.5f * (g[4] - g[2]);
-                g[1] = g[3] + (2.0f / 3.0f) * ru;
-                g[5] = g[7] + h + (1.0f / 6.0f) * ru;
-                g[8] = g[6] - h + (1.0f / 6.0f) * ru;
-            }
+            if (XEDGE == 1 && x == 0) zou_he_inlet<SC>(a, g);
         }
         float rho, ux, uy;
-        phys_moments<FORCE>(a, g, rho, ux, uy);
-        phys_collide<FORCE>(a, g, rho, ux, uy);
-        cell_store<T, XEDGE, FLAGS>(a, dst, nb, x, y, g);
+        phys_moments<FORCE, SC>(a, g, rho, ux, uy);
+        phys_collide<FORCE, SC>(a, g, rho, ux, uy);
+        cell_store<T, XEDGE, FLAGS, SC>(a, dst, nb, x, y, g);
     }
 }
 

@@ -19,6 +19,7 @@ import pytest
 from oracle import oracle as O
 from simuverse_amd import lbm as L
 from simuverse_amd import workloads as W
+from visible import visible_mask
 
 pytestmark = pytest.mark.gpu
 
@@ -125,6 +126,42 @@ def test_correction_conserves_mass_and_momentum(precision):
     plain.write_populations(f0, 0)
     plain.run(50)
     assert not np.array_equal(lat.read_populations(), plain.read_populations())
+
+
+@pytest.mark.parametrize("precision", [L.MLP_BF16, L.MLP_F32], ids=["bf16", "f32"])
+@pytest.mark.parametrize("x_edge", [L.X_PERIODIC, L.X_INLET_OUTLET], ids=["periodic", "channel"])
+def test_zero_network_step_equals_plain_step_f16(precision, x_edge):
+    """A network with all parameters zero adds nothing (y = 0, so every
+    modulation is 0), so the learned step, which collides unscaled binary16
+    deviations, must give the plain step's bits, which keeps them scaled by
+    2^10 through the step (power-of-two scaling rounds identically): channel
+    and periodic lattices with disks, Guo force, 120 steps."""
+    print(f"case: zero network vs plain, f16, {precision}, x_edge {x_edge}", flush=True)
+    nx, ny = 160, 96
+    if x_edge:
+        solid = W.cylinder_channel_solid(nx, ny, cx=40, cy=48, r=9)
+        kw = dict(x_edge=L.X_INLET_OUTLET, inlet_ux=0.05, force=(1e-6, 0.0))
+    else:
+        solid = W.periodic_disks(nx, ny, 0.05, 3, 8, seed=9).solid
+        kw = dict(force=(2e-6, -1e-6))
+    f0 = W.rest_state(nx, ny, solid, amplitude=1e-3, seed=4).astype(np.float32)
+    net = L.Network(seed=5)
+    net.set_params(np.zeros(5385, np.float32))
+    net.set_scales(1e-3, 1e-4)
+    out = []
+    for learned in (False, True):
+        lat = L.Lattice(nx, ny, 0.56, storage=L.STORAGE_F16_DEV, **kw)
+        lat.write_materials(W.materials_from_solid(solid))
+        lat.write_populations(f0, 0)
+        if learned:
+            lat.set_learned_collision(net)
+            lat.set_learned_precision(precision)
+        lat.run(120)
+        out.append(lat.read_populations())
+        lat.close()
+    vis = visible_mask(solid, periodic_x=not x_edge)
+    bad = np.count_nonzero(out[0][vis] != out[1][vis])
+    assert bad == 0, f"{bad} visible slots differ"
 
 
 def test_learned_step_bf16_tracks_f32():
