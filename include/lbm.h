@@ -312,8 +312,9 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* best_variant, double* ms_pe
  * x_i = (f_i - feq_i) / (w_i * in_scale), to y; its outputs y_0..y_2
  * modulate the non-equilibrium parts n_a of the second-order non-conserved
  * moments a = e, p_xx, p_xy (orthogonal moment basis) by alpha_a * n_a,
- * alpha_a = clamp(out_scale * y_a, -(2 - omega)/2, 0.5) (a bound that keeps
- * the relaxation stable), mapped back to populations (no mass, no
+ * alpha_a = clamp(out_scale * y_a, -(2 - omega)/2, (2 - omega)/2) (a bound
+ * that keeps the relaxation stable and adds no viscosity on average),
+ * mapped back to populations (no mass, no
  * momentum, no ghost moments) and added to the BGK post-collision
  * populations of cells whose 3x3 neighbourhood is fluid (the cells the
  * network is trained on).  Outputs 3..8 are unused (trained to zero).

@@ -601,15 +601,18 @@ __device__ inline void project_stress(float d[9])
 // Bounds of the learned modulation alpha of a moment's non-equilibrium part
 // (learned_correction): BGK leaves (1 - omega) n_a after the collision and
 // the correction adds alpha n_a, so |1 - omega + alpha| < 1 needs
-// alpha > omega - 2; the lower bound keeps half of that margin.
+// alpha > omega - 2; the lower bound keeps half of that margin, and the
+// upper bound mirrors it, so clipped noise in the network's output does not
+// shift the effective viscosity on average (near tau = 1/2 a small alpha
+// changes it a lot).
 __host__ __device__ inline float modulation_lo(float omega) { return -0.5f * (2.0f - omega); }
-constexpr float kModulationHi = 0.5f;
+__host__ __device__ inline float modulation_hi(float omega) { return 0.5f * (2.0f - omega); }
 
 // The learned collision correction of one cell.  The network's outputs
 // 0, 1, 2 modulate the non-equilibrium parts n_a of the second-order
 // non-conserved moments a = e, p_xx, p_xy of the cell (a learned,
 // state-dependent change of how far BGK relaxes each of them):
-//   dm_a = alpha_a n_a,   alpha_a = clamp(s_out y_a, modulation_lo, 0.5),
+//   dm_a = alpha_a n_a,   alpha_a = clamp(s_out y_a, modulation_lo, modulation_hi),
 //   n_a = sum_i m_a(i) (f_i - feq_i),
 // mapped back with the orthogonal basis (project_stress):
 //   corr_i = sum_a m_a(i) dm_a / |m_a|^2,
@@ -627,10 +630,10 @@ __device__ inline void learned_correction(const float xin[9], const float y[9], 
         nxx += (ex * ex - ey * ey) * q;
         nxy += (ex * ey) * q;
     }
-    const float lo = modulation_lo(omega);
-    const float ce = fminf(fmaxf(s_out * y[0], lo), kModulationHi) * ne * (1.0f / 36.0f);
-    const float cxx = fminf(fmaxf(s_out * y[1], lo), kModulationHi) * nxx * 0.25f;
-    const float cxy = fminf(fmaxf(s_out * y[2], lo), kModulationHi) * nxy * 0.25f;
+    const float lo = modulation_lo(omega), hi = modulation_hi(omega);
+    const float ce = fminf(fmaxf(s_out * y[0], lo), hi) * ne * (1.0f / 36.0f);
+    const float cxx = fminf(fmaxf(s_out * y[1], lo), hi) * nxx * 0.25f;
+    const float cxy = fminf(fmaxf(s_out * y[2], lo), hi) * nxy * 0.25f;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const float ex = (float)ex_of(i), ey = (float)ey_of(i);

@@ -345,8 +345,8 @@ __global__ __launch_bounds__(256) void k_mlp_transition(const float* __restrict_
         dxy += (ex * ey) * d[i];
     }
     const float eps2 = 1e-2f * (nxx * nxx + nxy * nxy + ne * ne * (1.0f / 9.0f)) + 1e-30f;
-    const float lo = modulation_lo(a.omega);
-    auto ratio = [&](float dm, float nm) { return fminf(fmaxf(dm * nm / (nm * nm + eps2), lo), kModulationHi); };
+    const float lo = modulation_lo(a.omega), hi = modulation_hi(a.omega);
+    auto ratio = [&](float dm, float nm) { return fminf(fmaxf(dm * nm / (nm * nm + eps2), lo), hi); };
     float t[9] = {ratio(de, ne), ratio(dxx, nxx), ratio(dxy, nxy), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     const int slot = atomicAdd(count, 1);
     if (slot >= max_samples) return;
