@@ -26,6 +26,10 @@ extern const StepVariant kStepVariants[];
 extern const int kNumStepVariants;
 bool step_variant_ok(int variant, int nx, int ny, int xedge);
 inline bool step_variant_two_step(int variant) { return (kStepVariants[variant].flags & kTwoStep) != 0; }
+// Whether two-step tiles of width bw fit an nx x ny lattice (lbm_step2.hip).
+bool two_step_fits(int bw, int nx, int ny, int xedge);
+// Blocks of bs threads for n work items.
+inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // Words per row of the packed solid mask (bit x+1 of a row is cell x).
 inline uint32_t solid_mask_words(int nx) { return (((uint32_t)nx + 2u) / 32u + 2u + 1u) & ~1u; }
