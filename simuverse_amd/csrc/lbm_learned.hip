@@ -183,9 +183,15 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
         }
         *reinterpret_cast<bf16x8*>(&xs[wave][lane * 16]) = lo;
         *reinterpret_cast<bf16x8*>(&xs[wave][lane * 16 + 8]) = hi;
-        __syncthreads();
+        // xs[wave] and ys[wave] belong to this wavefront alone, whose LDS
+        // accesses complete in issue order: a wavefront-scope fence keeps
+        // the compiler from moving them across, and no block barrier is
+        // needed (the wavefronts of a block run independently).
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
         mlp_wave_bf16(W, bias, xs[wave], ys[wave], lane);
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
         if (fluid) {
             float corr[9], y9[9];
 #pragma unroll
