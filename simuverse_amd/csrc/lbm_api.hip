@@ -709,7 +709,8 @@ int lbm_set_variant(lbm_handle* h, int variant)
     if (rc) return rc;
     if (variant < 0 || variant >= kNumStepVariants) return fail(LBM_ERR_INVALID, "no variant %d", variant);
     if (!step_variant_ok(variant, h->d.nx, h->d.ny, h->d.x_edge))
-        return fail(LBM_ERR_INVALID, "variant %d needs nx divisible by %d", variant, kStepVariants[variant].vec);
+        return fail(LBM_ERR_INVALID, "variant %d does not fit a %dx%d lattice with x edge %d (see lbm_variant_info)",
+                    variant, h->d.nx, h->d.ny, h->d.x_edge);
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (variant != h->variant) drop_graphs(h);
     h->variant = variant;
