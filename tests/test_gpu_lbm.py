@@ -357,22 +357,37 @@ def test_f16_deviation_storage_vs_fp64():
     assert e["rho_l2"] < 1e-5 and e["u_l2"] < 5e-2
 
 
-def test_full_size_4096_mass_and_momentum_properties():
-    """The 4096^2 roofline lattice (periodic, disks >= 5 %, g = 1e-6,
-    tau = 0.55): the binary64 reduction shows mass conserved and the
-    momentum identity, 100 steps."""
-    print("case: 4096x4096 properties 100 steps", flush=True)
-    n = 4096
+@pytest.mark.parametrize("n, storage, variant", [(4096, L.STORAGE_F32, None), (4096, L.STORAGE_F32, 24),
+                                               (8192, L.STORAGE_F16_DEV, 26)],
+                         ids=["4096-f32-default", "4096-f32-two-step", "8192-f16-two-step"])
+def test_full_size_mass_and_momentum_properties(n, storage, variant):
+    """The roofline lattices at full size (periodic, disks >= 5 %,
+    g = 1e-6, tau = 0.55; 4096^2 binary32 with the default one-step shape and
+    the two-step shape the benchmark tunes to, 8192^2 binary16 deviations
+    with its two-step shape): the binary64 reduction shows mass conserved
+    over 100 steps and the momentum identity P(t+1) - P(t) = g N - drag for
+    one more step.  The identity holds to the rounding of the stored
+    populations: measured 1.2e-6 of g N in binary32 (tolerance 1e-3) and
+    2.2e-3 of g N in binary16 (tolerance 1e-2), where the nearly uniform
+    early deviations (about 3 w g per step) round the same way in every
+    cell, so their rounding errors add up instead of cancelling."""
+    print(f"case: {n}x{n} properties 100 steps, storage {storage}, variant {variant}", flush=True)
     scene = W.periodic_disks(n, n)
     g = 1e-6
-    lat = L.Lattice(n, n, 0.55, force=(g, 0.0))
+    lat = L.Lattice(n, n, 0.55, force=(g, 0.0), storage=storage)
     lat.write_materials(W.materials_from_solid(scene.solid))
     lat.reset()
+    if variant is not None:
+        lat.set_variant(variant)
     nfl = int((~scene.solid).sum())
     r0 = lat.reduce()
     lat.run(100)
     r1 = lat.reduce()
+    assert np.isfinite(r1).all()
     assert abs(r1[0] - r0[0]) < 1e-6 * r0[0]
     lat.run(1)
     r2 = lat.reduce()
-    assert abs((r2[1] - r1[1]) - (g * nfl - r2[3])) < 1e-3 * g * nfl
+    print(f"  mass drift {abs(r1[0] - r0[0]) / r0[0]:.2e}, momentum identity residual "
+          f"{abs((r2[1] - r1[1]) - (g * nfl - r2[3])) / (g * nfl):.2e}", flush=True)
+    tol = 1e-3 if storage == L.STORAGE_F32 else 1e-2
+    assert abs((r2[1] - r1[1]) - (g * nfl - r2[3])) < tol * g * nfl
