@@ -38,6 +38,12 @@ HIPCC_FLAGS = [
     "-Wall",
     "-Wno-unused-function",
 ]
+# Per-file additions.  The learned step is bound by vector instructions
+# around its matrix instructions; with the accumulators in ordinary VGPRs
+# (rather than AGPRs) it needs no copies between the two register files.
+EXTRA_FLAGS = {
+    "lbm_learned.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+}
 
 
 def hipcc() -> str:
@@ -57,7 +63,8 @@ def build(verbose: bool = False) -> Path:
     # the translation units compile in parallel (one hipcc each)
     for src in SOURCES:
         obj = objdir / (Path(src).stem + ".o")
-        cmd = [cc, *HIPCC_FLAGS, "-I", str(ROOT / "include"), "-I", str(CSRC), "-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [cc, *HIPCC_FLAGS, *EXTRA_FLAGS.get(src, []), "-I", str(ROOT / "include"), "-I", str(CSRC), "-c",
+               str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd)))

@@ -345,16 +345,19 @@ __device__ inline void phys_collide(const PhysArgs& a, float g[9], float rho, fl
 }
 
 // The learned correction's input: the non-equilibrium part g - geq = f - feq
-// over w_i s_in.
+// over w_i s_in, as a product with the reciprocal (1/w_i is exact: 9/4, 9,
+// 36; one division per cell instead of nine).
+__host__ __device__ constexpr float inv_w_phys(int i) { return i == 0 ? 2.25f : (i <= 4 ? 9.0f : 36.0f); }
 __device__ __forceinline__ void cell_features(const float g[9], float rho, float ux, float uy, float s_in,
                                               float xin[9])
 {
     const float dr = rho - 1.0f, usq = 1.5f * (ux * ux + uy * uy);
+    const float inv_s = 1.0f / s_in;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const float eu = (float)ex_of(i) * ux + (float)ey_of(i) * uy;
         const float geq = w_phys(i) * (dr + rho * (3.0f * eu + 4.5f * (eu * eu) - usq));
-        xin[i] = (g[i] - geq) / (w_phys(i) * s_in);
+        xin[i] = (g[i] - geq) * (inv_w_phys(i) * inv_s);
     }
 }
 
@@ -467,21 +470,49 @@ __host__ __device__ inline float pack_weight(const float* P, int e)
     return o < kMlpOut ? P[kOffW3 + o * kMlpH + chain_unit(k)] : 0.0f;
 }
 
-// GELU as x * Phi(x) with Phi(x) = clamp(0.5 + x c(x^2), 0, 1) on
-// x clamped to [-3.5, 3.5], c a degree-4 polynomial in x^2 whose
-// coefficients were fitted to minimise the largest deviation from the erf
-// form over [-10, 10]: 1.2e-3 absolute, below the bf16 rounding of the
-// activations it feeds.  Nine vector instructions, no transcendentals.
-__device__ __forceinline__ float gelu_poly(float x)
+// GELU of two activations as x * Phi(x), Phi(x) = clamp(0.5 + x c(x^2), 0, 1),
+// c a degree-4 polynomial in x^2 fitted to minimise the largest deviation
+// from the erf form over [-10, 10].  Phi is evaluated in packed binary16
+// (two activations per instruction: v_cvt_pkrtz_f16_f32, v_pk_mul_f16,
+// five v_pk_fma_f16, the last with the [0, 1] clamp) and multiplies the
+// binary32 x (v_fma_mix_f32): ten vector instructions per pair of
+// activations, with the bf16 packing.  |GELU error| <= 5.2e-3 (at x = 3.35, where half a bf16 ulp
+// of the activation is 7.8e-3), rms 9e-4 over |x| < 3.  The argument needs
+// no clamp: checked over every finite binary16 x (binary16 coefficients,
+// subnormal leading one included), Phi comes out exactly 1 for x >= 3.5
+// and exactly 0 for x <= -3.5 -- x c(x^2) rises monotonically there, and
+// overflow of x^2 or c gives +inf, which the clamp maps correctly.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ f16x2 f16_splat(float v) { return (f16x2){(_Float16)v, (_Float16)v}; }
+// Eight activations (four independent pairs, stage by stage, so dependent
+// packed instructions are never adjacent) to bf16.
+__device__ __forceinline__ bf16x8 gelu8_bf16(const float* v)
 {
-    const float xc = fminf(fmaxf(x, -3.5f), 3.5f);
-    const float x2 = xc * xc;
-    float c = fmaf(1.02411032e-05f, x2, -4.08110474e-04f);
-    c = fmaf(c, x2, 6.66348797e-03f);
-    c = fmaf(c, x2, -5.97102903e-02f);
-    c = fmaf(c, x2, 3.94299256e-01f);
-    const float phi = fminf(fmaxf(fmaf(c, xc, 0.5f), 0.0f), 1.0f);
-    return x * phi;
+    f16x2 x[4], x2[4], c[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) x[p] = __builtin_bit_cast(f16x2, __builtin_amdgcn_cvt_pkrtz(v[2 * p], v[2 * p + 1]));
+#pragma unroll
+    for (int p = 0; p < 4; ++p) x2[p] = x[p] * x[p];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) c[p] = __builtin_elementwise_fma(x2[p], f16_splat(1.02411032e-05f), f16_splat(-4.08110474e-04f));
+#pragma unroll
+    for (int p = 0; p < 4; ++p) c[p] = __builtin_elementwise_fma(c[p], x2[p], f16_splat(6.66348797e-03f));
+#pragma unroll
+    for (int p = 0; p < 4; ++p) c[p] = __builtin_elementwise_fma(c[p], x2[p], f16_splat(-5.97102903e-02f));
+#pragma unroll
+    for (int p = 0; p < 4; ++p) c[p] = __builtin_elementwise_fma(c[p], x2[p], f16_splat(3.94299256e-01f));
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        c[p] = __builtin_elementwise_fma(c[p], x[p], f16_splat(0.5f));
+        c[p] = __builtin_elementwise_min(__builtin_elementwise_max(c[p], f16_splat(0.0f)), f16_splat(1.0f));
+    }
+    bf16x8 out;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        out[2 * p] = (__bf16)__builtin_fmaf(v[2 * p], (float)c[p].x, 0.0f);
+        out[2 * p + 1] = (__bf16)__builtin_fmaf(v[2 * p + 1], (float)c[p].y, 0.0f);
+    }
+    return out;
 }
 
 __device__ __forceinline__ bf16x8 lds_b128(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
@@ -507,8 +538,13 @@ __device__ __forceinline__ void mlp_wave_bf16(const __bf16* __restrict__ W, cons
 #pragma unroll
             for (int j = 0; j < 16; ++j) c[j] = b1[32 * mt + c_row(j, h)];
             c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b128(W + kPackW1 + (mt * 64 + lane) * 8), xb, c, 0, 0, 0);
+{
+                float v[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) hb[2 * mt + (j >> 3)][j & 7] = (__bf16)gelu_poly(c[j]);
+                for (int j = 0; j < 16; ++j) v[j] = c[j];
+                hb[2 * mt] = gelu8_bf16(v);
+                hb[2 * mt + 1] = gelu8_bf16(v + 8);
+            }
         }
         bf16x8 hb2[4];
 #pragma unroll
@@ -520,8 +556,13 @@ __device__ __forceinline__ void mlp_wave_bf16(const __bf16* __restrict__ W, cons
             for (int s = 0; s < 4; ++s)
                 c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b128(W + kPackW2 + ((mt * 4 + s) * 64 + lane) * 8),
                                                             hb[s], c, 0, 0, 0);
+{
+                float v[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) hb2[2 * mt + (j >> 3)][j & 7] = (__bf16)gelu_poly(c[j]);
+                for (int j = 0; j < 16; ++j) v[j] = c[j];
+                hb2[2 * mt] = gelu8_bf16(v);
+                hb2[2 * mt + 1] = gelu8_bf16(v + 8);
+            }
         }
         f32x16 c;
 #pragma unroll
@@ -533,20 +574,15 @@ __device__ __forceinline__ void mlp_wave_bf16(const __bf16* __restrict__ W, cons
         for (int s = 0; s < 4; ++s)
             c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b128(W + kPackW3 + (s * 64 + lane) * 8), hb2[s], c, 0, 0,
                                                         0);
-        // Outputs 0..8 are rows 0-3 and 8 (lane half 0) and 4-7 (half 1).
-        float* y = ys + (32 * nt + n) * kMlpOut;
-        if (h == 0) {
-            y[0] = c[0];
-            y[1] = c[1];
-            y[2] = c[2];
-            y[3] = c[3];
-            y[8] = c[4];
-        } else {
-            y[4] = c[0];
-            y[5] = c[1];
-            y[6] = c[2];
-            y[7] = c[3];
-        }
+        // Outputs 0..8 are rows 0-3 and 8 (lane half 0) and 4-7 (half 1):
+        // both halves store c[0..3] at outputs 4h..4h+3 (the same register
+        // indices in every lane, so no per-lane register selection).
+        float* y = ys + (32 * nt + n) * kMlpOut + 4 * h;
+        y[0] = c[0];
+        y[1] = c[1];
+        y[2] = c[2];
+        y[3] = c[3];
+        if (h == 0) y[8] = c[4];
     }
 }
 
@@ -617,9 +653,10 @@ __host__ __device__ inline float modulation_hi(float omega) { return 0.5f * (2.0
 // mapped back with the orthogonal basis (project_stress):
 //   corr_i = sum_a m_a(i) dm_a / |m_a|^2,
 // so the correction carries no mass, no momentum and no ghost moments.
-// xin holds the network inputs (f_i - feq_i) / (w_i s_in).
-__device__ inline void learned_correction(const float xin[9], const float y[9], float s_in, float s_out, float omega,
-                                          float corr[9])
+// xin holds the network inputs (f_i - feq_i) / (w_i s_in).  The two halves
+// are separate so a kernel can form n_a before the network runs and keep
+// three values, not nine, live across it.
+__device__ inline void stress_neq(const float xin[9], float s_in, float n[3])
 {
     float ne = 0.0f, nxx = 0.0f, nxy = 0.0f;
 #pragma unroll
@@ -630,15 +667,28 @@ __device__ inline void learned_correction(const float xin[9], const float y[9], 
         nxx += (ex * ex - ey * ey) * q;
         nxy += (ex * ey) * q;
     }
+    n[0] = ne;
+    n[1] = nxx;
+    n[2] = nxy;
+}
+__device__ inline void stress_correction(const float n[3], const float y[3], float s_out, float omega, float corr[9])
+{
     const float lo = modulation_lo(omega), hi = modulation_hi(omega);
-    const float ce = fminf(fmaxf(s_out * y[0], lo), hi) * ne * (1.0f / 36.0f);
-    const float cxx = fminf(fmaxf(s_out * y[1], lo), hi) * nxx * 0.25f;
-    const float cxy = fminf(fmaxf(s_out * y[2], lo), hi) * nxy * 0.25f;
+    const float ce = fminf(fmaxf(s_out * y[0], lo), hi) * n[0] * (1.0f / 36.0f);
+    const float cxx = fminf(fmaxf(s_out * y[1], lo), hi) * n[1] * 0.25f;
+    const float cxy = fminf(fmaxf(s_out * y[2], lo), hi) * n[2] * 0.25f;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const float ex = (float)ex_of(i), ey = (float)ey_of(i);
         corr[i] = mom_e(i) * ce + (ex * ex - ey * ey) * cxx + (ex * ey) * cxy;
     }
+}
+__device__ inline void learned_correction(const float xin[9], const float y[9], float s_in, float s_out, float omega,
+                                          float corr[9])
+{
+    float n[3];
+    stress_neq(xin, s_in, n);
+    stress_correction(n, y, s_out, omega, corr);
 }
 
 // Linear block id -> logical tile id such that the blocks the hardware

@@ -170,10 +170,17 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
             learned = (nb.sm | nb.s0 | nb.sp) == 0u && (XEDGE == 0 || (x > 0 && x < a.nx - 1));
         }
         float xin[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        float nneq[3] = {0.0f, 0.0f, 0.0f};
         if (fluid) {
             cell_gather<T, XEDGE, kNtStore>(a, src, nb, x, y, g);
             phys_moments<FORCE>(a, g, rho, ux, uy);
-            if (learned) cell_features(g, rho, ux, uy, s_in, xin);
+            if (learned) {
+                cell_features(g, rho, ux, uy, s_in, xin);
+                stress_neq(xin, s_in, nneq);
+            }
+            // BGK now, the correction after the network: only g, the three
+            // moments and the neighbour bits stay live across it.
+            phys_collide<FORCE>(a, g, rho, ux, uy);
         }
         bf16x8 lo, hi;
 #pragma unroll
@@ -193,12 +200,11 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         if (fluid) {
-            float corr[9], y9[9];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) y9[i] = ys[wave][lane * kMlpOut + i];
-            learned_correction(xin, y9, s_in, s_out, a.omega, corr);
-            phys_collide<FORCE>(a, g, rho, ux, uy);
             if (learned) {
+                float corr[9], y3[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) y3[i] = ys[wave][lane * kMlpOut + i];
+                stress_correction(nneq, y3, s_out, a.omega, corr);
 #pragma unroll
                 for (int i = 0; i < 9; ++i) g[i] += corr[i];
             }

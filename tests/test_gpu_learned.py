@@ -89,6 +89,33 @@ def test_forward_bf16_matrix_cores_vs_fp64_oracle():
     assert np.array_equal(net.forward(x, bf16=True), np.zeros_like(x))
 
 
+def test_forward_bf16_large_activations_vs_fp64_oracle():
+    """Matrix-core forward with hidden pre-activations spread over
+    |x| ~ 1e-1 .. 9e3 (median 50 in layer 1, 770 in layer 2, 81 % of layer 2
+    beyond 256): the GELU saturates (exactly x or 0 for |x| >= 3.5) and the
+    binary16 evaluation of its polynomial overflows (x^2 and the polynomial
+    become +inf for |x| >= 256), which must still give x or 0.  Same bounds
+    as the unscaled forward."""
+    print("case: network forward bf16, large activations, vs fp64", flush=True)
+    net = L.Network(seed=3)
+    p = random_params(1, 0.15).astype(np.float64)
+    p[:640] *= 200.0  # W1 and b1
+    p[640:4800] *= 20.0  # W2 and b2
+    p = p.astype(np.float32)
+    net.set_params(p)
+    x = np.random.default_rng(2).standard_normal((1000, 9)).astype(np.float32)
+    x[::7] *= 1e-3  # and some near zero
+    y = net.forward(x, bf16=True)
+    ref = O.mlp_forward(p.astype(np.float64), x.astype(np.float64))
+    err = y - ref
+    rms = float(np.sqrt(np.mean(ref**2)))
+    print(f"bf16 forward (large): rms err {np.sqrt(np.mean(err**2)) / rms:.3e}, "
+          f"max err {np.max(np.abs(err)) / np.max(np.abs(ref)):.3e}", flush=True)
+    assert np.all(np.isfinite(y))
+    assert np.sqrt(np.mean(err**2)) < 1e-2 * rms
+    assert np.max(np.abs(err)) < 3e-2 * np.max(np.abs(ref))
+
+
 def test_training_reduces_loss():
     print("case: network training", flush=True)
     rng = np.random.default_rng(6)
