@@ -102,19 +102,27 @@ __global__ __launch_bounds__(256) void k_mlp_forward_bf16(const __bf16* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Loss gradient: a block of 256 threads takes chunks of 64 samples (chunks
+// Loss gradient: a block of 1024 threads takes chunks of 64 samples (chunks
 // b, b + grid, ...); for a chunk, thread (s, q) = (tid % 64, tid / 64)
-// computes hidden units 16q..16q+15 of sample s, activations are staged in
-// LDS (rows padded to 65 floats), and each thread adds the chunk's
-// sample-sums of its parameters p = tid + 256 r to registers.  One partial
-// per block, summed over the (at most kGradBlocks) blocks in block order.
+// computes hidden units 4q..4q+3 of sample s, activations are staged in LDS
+// (rows padded to 65 floats), and each thread adds the chunk's sample-sums
+// of its parameters p = tid + 1024 r to registers.  One partial per block,
+// summed over the (at most kGradBlocks) blocks in block order.  The LDS
+// (about 130 KiB) admits one block per CU, so the block is as large as a
+// block can be: four wavefronts per SIMD hide each other's LDS latency.
+// Every sum runs in the same order for any block size (per parameter:
+// samples in chunk order, t = 0..63 within a chunk).
 // ---------------------------------------------------------------------------
 constexpr int kTS = 64;       // samples per chunk
 constexpr int kPadH = kMlpH + 1;
 constexpr int kGradBlocks = 1024;
-constexpr int kPerThread = (kMlpParams + 255) / 256;
+constexpr int kGradThreads = 1024;
+constexpr int kUnitGroups = kGradThreads / kTS;  // 16
+constexpr int kUnitsPerThread = kMlpH / kUnitGroups;  // 4
+constexpr int kPerThread = (kMlpParams + kGradThreads - 1) / kGradThreads;
+static_assert(kUnitGroups >= kMlpOut, "one output per unit group");
 
-__global__ __launch_bounds__(256) void k_mlp_grad_partial(const float* __restrict__ params,
+__global__ __launch_bounds__(kGradThreads) void k_mlp_grad_partial(const float* __restrict__ params,
                                                           const float* __restrict__ X, const float* __restrict__ T,
                                                           int n, int batch, uint64_t seed, int use_index,
                                                           float* __restrict__ partial, double* __restrict__ loss_part)
@@ -123,9 +131,9 @@ __global__ __launch_bounds__(256) void k_mlp_grad_partial(const float* __restric
     __shared__ float xs[kTS][kMlpIn], dy[kTS][kMlpOut];
     __shared__ float A1[kTS][kPadH], H1[kTS][kPadH], A2[kTS][kPadH], H2[kTS][kPadH], D1[kTS][kPadH],
         D2[kTS][kPadH];
-    __shared__ double lsum[kTS];
+    __shared__ double lsq[kTS][kMlpOut];
     const int tid = threadIdx.x;
-    for (int k = tid; k < kMlpParams; k += 256) P[k] = params[k];
+    for (int k = tid; k < kMlpParams; k += kGradThreads) P[k] = params[k];
     const int s = tid & (kTS - 1), q = tid >> 6;
     float acc[kPerThread];
 #pragma unroll
@@ -144,8 +152,8 @@ __global__ __launch_bounds__(256) void k_mlp_grad_partial(const float* __restric
     }
     __syncthreads();
     // layer 1
-    for (int jj = 0; jj < 16; ++jj) {
-        const int j = 16 * q + jj;
+    for (int jj = 0; jj < kUnitsPerThread; ++jj) {
+        const int j = kUnitsPerThread * q + jj;
         float a = P[kOffB1 + j];
 #pragma unroll
         for (int i = 0; i < kMlpIn; ++i) a += P[kOffW1 + j * kMlpIn + i] * xs[s][i];
@@ -154,40 +162,39 @@ __global__ __launch_bounds__(256) void k_mlp_grad_partial(const float* __restric
     }
     __syncthreads();
     // layer 2
-    for (int jj = 0; jj < 16; ++jj) {
-        const int j = 16 * q + jj;
+    for (int jj = 0; jj < kUnitsPerThread; ++jj) {
+        const int j = kUnitsPerThread * q + jj;
         float a = P[kOffB2 + j];
+#pragma unroll 8
         for (int k = 0; k < kMlpH; ++k) a += P[kOffW2 + j * kMlpH + k] * H1[s][k];
         A2[s][j] = a;
         H2[s][j] = gelu(a);
     }
     __syncthreads();
-    // output, loss and dL/dy
-    if (q == 0) {
-        double l = 0.0;
+    // output m = q, its squared residual and dL/dy
+    if (q < kMlpOut) {
+        const int m = q;
         const float inv = 2.0f / (9.0f * (float)batch);
-#pragma unroll
-        for (int m = 0; m < kMlpOut; ++m) {
-            float y = P[kOffB3 + m];
-            for (int k = 0; k < kMlpH; ++k) y += P[kOffW3 + m * kMlpH + k] * H2[s][k];
-            const float r = live ? y - T[(size_t)idx * kMlpOut + m] : 0.0f;
-            l += (double)r * (double)r;
-            dy[s][m] = r * inv;
-        }
-        lsum[s] = l;
+        float y = P[kOffB3 + m];
+#pragma unroll 8
+        for (int k = 0; k < kMlpH; ++k) y += P[kOffW3 + m * kMlpH + k] * H2[s][k];
+        const float r = live ? y - T[(size_t)idx * kMlpOut + m] : 0.0f;
+        lsq[s][m] = (double)r * (double)r;
+        dy[s][m] = r * inv;
     }
     __syncthreads();
-    for (int jj = 0; jj < 16; ++jj) {
-        const int j = 16 * q + jj;
+    for (int jj = 0; jj < kUnitsPerThread; ++jj) {
+        const int j = kUnitsPerThread * q + jj;
         float d = 0.0f;
 #pragma unroll
         for (int m = 0; m < kMlpOut; ++m) d += P[kOffW3 + m * kMlpH + j] * dy[s][m];
         D2[s][j] = d * gelu_grad(A2[s][j]);
     }
     __syncthreads();
-    for (int jj = 0; jj < 16; ++jj) {
-        const int j = 16 * q + jj;
+    for (int jj = 0; jj < kUnitsPerThread; ++jj) {
+        const int j = kUnitsPerThread * q + jj;
         float d = 0.0f;
+#pragma unroll 8
         for (int k = 0; k < kMlpH; ++k) d += P[kOffW2 + k * kMlpH + j] * D2[s][k];
         D1[s][j] = d * gelu_grad(A1[s][j]);
     }
@@ -195,37 +202,47 @@ __global__ __launch_bounds__(256) void k_mlp_grad_partial(const float* __restric
     // this chunk's sample-sums of the weight gradients, into registers
 #pragma unroll
     for (int r = 0; r < kPerThread; ++r) {
-        const int p = tid + 256 * r;
+        const int p = tid + kGradThreads * r;
         if (p >= kMlpParams) break;
         float c = 0.0f;
         if (p < kOffB1) {
             const int j = p / kMlpIn, i = p % kMlpIn;
+#pragma unroll 8
             for (int t = 0; t < kTS; ++t) c += D1[t][j] * xs[t][i];
         } else if (p < kOffW2) {
             const int j = p - kOffB1;
+#pragma unroll 8
             for (int t = 0; t < kTS; ++t) c += D1[t][j];
         } else if (p < kOffB2) {
             const int j = (p - kOffW2) / kMlpH, k = (p - kOffW2) % kMlpH;
+#pragma unroll 8
             for (int t = 0; t < kTS; ++t) c += D2[t][j] * H1[t][k];
         } else if (p < kOffW3) {
             const int j = p - kOffB2;
+#pragma unroll 8
             for (int t = 0; t < kTS; ++t) c += D2[t][j];
         } else if (p < kOffB3) {
             const int m = (p - kOffW3) / kMlpH, k = (p - kOffW3) % kMlpH;
+#pragma unroll 8
             for (int t = 0; t < kTS; ++t) c += dy[t][m] * H2[t][k];
         } else {
             const int m = p - kOffB3;
+#pragma unroll 8
             for (int t = 0; t < kTS; ++t) c += dy[t][m];
         }
         acc[r] += c;
     }
     if (tid == 0)
-        for (int t = 0; t < kTS; ++t) lblock += lsum[t];
+        for (int t = 0; t < kTS; ++t) {
+            double l = 0.0;
+            for (int m = 0; m < kMlpOut; ++m) l += lsq[t][m];
+            lblock += l;
+        }
     }  // chunks
     float* out = partial + (size_t)blockIdx.x * kMlpParams;
 #pragma unroll
     for (int r = 0; r < kPerThread; ++r) {
-        const int p = tid + 256 * r;
+        const int p = tid + kGradThreads * r;
         if (p < kMlpParams) out[p] = acc[r];
     }
     if (tid == 0) loss_part[blockIdx.x] = lblock;
@@ -291,25 +308,59 @@ __global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ p, const f
 //   target    t_i = P(f_i(c + e_i, t+1) - BGK_i(f(c, t))) / (w_i s_out)
 // where P removes the mass and momentum of a correction.  Cells that are
 // solid or have a solid neighbour are skipped; y and x are periodic.
+// Candidate k is cell k * stride + offset.  The samples are stored in
+// candidate order (a count pass, an exclusive scan of the per-block counts,
+// then each block writes at its base in thread order), so the sample set,
+// and the training that draws from it, are the same on every run.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_mlp_transition(const float* __restrict__ s0, const float* __restrict__ s1,
-                                                        const uint8_t* __restrict__ solid, PhysArgs a, float s_in,
-                                                        float s_out, uint32_t stride, uint32_t offset,
-                                                        int max_samples, int* __restrict__ count,
-                                                        float* __restrict__ X, float* __restrict__ T)
+__device__ inline bool transition_cell(const uint8_t* __restrict__ solid, int nx, int ny, uint32_t k,
+                                       uint32_t stride, uint32_t offset, uint32_t& c)
 {
-    const int nx = a.nx, ny = a.ny;
     const uint32_t n = (uint32_t)nx * (uint32_t)ny;
-    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
     const uint64_t c64 = (uint64_t)k * stride + offset;
-    if (c64 >= n) return;
-    const uint32_t c = (uint32_t)c64;
+    if (c64 >= n) return false;
+    c = (uint32_t)c64;
     const int y = (int)(c / (uint32_t)nx), x = (int)(c % (uint32_t)nx);
     for (int dy2 = -1; dy2 <= 1; ++dy2)
         for (int dx2 = -1; dx2 <= 1; ++dx2) {
             const int xx = (x + dx2 + nx) % nx, yy = (y + dy2 + ny) % ny;
-            if (solid[(uint32_t)yy * nx + xx]) return;
+            if (solid[(uint32_t)yy * nx + xx]) return false;
         }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_transition_count(const uint8_t* __restrict__ solid, int nx, int ny,
+                                                              uint32_t stride, uint32_t offset, int* __restrict__ counts)
+{
+    uint32_t c = 0;
+    const bool take = transition_cell(solid, nx, ny, blockIdx.x * 256u + threadIdx.x, stride, offset, c);
+    const int total = __syncthreads_count(take ? 1 : 0);
+    if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+// block_base: first slot of each block's samples (the running sample count
+// plus the exclusive scan of the counts, clamped to max_samples).
+__global__ __launch_bounds__(256) void k_mlp_transition(const float* __restrict__ s0, const float* __restrict__ s1,
+                                                        const uint8_t* __restrict__ solid, PhysArgs a, float s_in,
+                                                        float s_out, uint32_t stride, uint32_t offset,
+                                                        int max_samples, const int* __restrict__ block_base,
+                                                        float* __restrict__ X, float* __restrict__ T)
+{
+    __shared__ int wave_total[4];
+    const int nx = a.nx, ny = a.ny;
+    const uint32_t n = (uint32_t)nx * (uint32_t)ny;
+    uint32_t c = 0;
+    const bool take = transition_cell(solid, nx, ny, blockIdx.x * 256u + threadIdx.x, stride, offset, c);
+    // rank of this cell among the block's samples, in thread order
+    const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+    const uint64_t mask = __ballot(take ? 1 : 0);
+    const int below = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_total[wave] = __popcll(mask);
+    __syncthreads();
+    if (!take) return;
+    int rank = below;
+    for (int w = 0; w < wave; ++w) rank += wave_total[w];
+    const int y = (int)(c / (uint32_t)nx), x = (int)(c % (uint32_t)nx);
     // The step's own arithmetic on deviations g = f - w: moments, features,
     // BGK (+ Guo).
     float g[9], gc[9];
@@ -348,7 +399,7 @@ __global__ __launch_bounds__(256) void k_mlp_transition(const float* __restrict_
     const float lo = modulation_lo(a.omega), hi = modulation_hi(a.omega);
     auto ratio = [&](float dm, float nm) { return fminf(fmaxf(dm * nm / (nm * nm + eps2), lo), hi); };
     float t[9] = {ratio(de, ne), ratio(dxx, nxx), ratio(dxy, nxy), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    const int slot = atomicAdd(count, 1);
+    const int slot = block_base[blockIdx.x] + rank;
     if (slot >= max_samples) return;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -373,7 +424,6 @@ struct lbm_mlp {
     int partial_blocks = 0;
     float* X = nullptr;  // samples
     float* T = nullptr;
-    int* count = nullptr;
     int capacity = 0;
     int samples = 0;
     int adam_step = 0;
@@ -392,7 +442,7 @@ int mlp_check(lbm_mlp* m)
 void mlp_free(lbm_mlp* m)
 {
     for (void* p : {(void*)m->params, (void*)m->packed, (void*)m->adam_m, (void*)m->adam_v, (void*)m->grad, (void*)m->partial,
-                    (void*)m->loss_part, (void*)m->loss, (void*)m->X, (void*)m->T, (void*)m->count})
+                    (void*)m->loss_part, (void*)m->loss, (void*)m->X, (void*)m->T})
         if (p) (void)hipFree(p);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
@@ -412,7 +462,7 @@ int mlp_grad(lbm_mlp* m, const float* X, const float* T, int n, int batch, uint6
         MLP_TRY(hipMalloc(&m->loss_part, (size_t)nb * sizeof(double)));
         m->partial_blocks = nb;
     }
-    hipLaunchKernelGGL(k_mlp_grad_partial, dim3(nb), dim3(256), 0, m->stream, m->params, X, T, n, batch, seed,
+    hipLaunchKernelGGL(k_mlp_grad_partial, dim3(nb), dim3(kGradThreads), 0, m->stream, m->params, X, T, n, batch, seed,
                        use_index, m->partial, m->loss_part);
     MLP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_mlp_grad_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, m->stream, m->partial, nb,
@@ -462,7 +512,6 @@ int lbm_mlp_create(uint64_t seed, int capacity, lbm_mlp** out)
     if (e == hipSuccess) e = hipMalloc(&m->adam_v, pb);
     if (e == hipSuccess) e = hipMalloc(&m->grad, pb);
     if (e == hipSuccess) e = hipMalloc(&m->loss, sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&m->count, sizeof(int));
     if (e == hipSuccess && capacity > 0) e = hipMalloc(&m->X, (size_t)capacity * kMlpIn * sizeof(float));
     if (e == hipSuccess && capacity > 0) e = hipMalloc(&m->T, (size_t)capacity * kMlpOut * sizeof(float));
     if (e != hipSuccess) {
@@ -686,32 +735,51 @@ int lbm_mlp_add_transition(lbm_mlp* m, const float* state_t, const float* state_
     if (e == hipSuccess) e = hipMemcpyAsync(s0, state_t, 9 * n * sizeof(float), hipMemcpyHostToDevice, m->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s1, state_t1, 9 * n * sizeof(float), hipMemcpyHostToDevice, m->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(sd, solid, n, hipMemcpyHostToDevice, m->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(m->count, &m->samples, sizeof(int), hipMemcpyHostToDevice, m->stream);
     const size_t cand = (n - (size_t)std::min<int>(offset, (int)n) + stride - 1) / stride;
-    if (e == hipSuccess && cand > 0) {
-        PhysArgs pa{};
-        pa.nx = nx;
-        pa.ny = ny;
-        pa.omega = 1.0f / tau;
-        pa.gx = force_x;
-        pa.gy = force_y;
-        pa.scale = pa.inv_scale = 1.0f;
-        set_force_terms(pa);
-        hipLaunchKernelGGL(k_mlp_transition, dim3((unsigned)((cand + 255) / 256)), dim3(256), 0, m->stream, s0, s1, sd,
-                           pa, m->s_in, m->s_out, (uint32_t)stride, (uint32_t)offset, m->capacity, m->count, m->X,
-                           m->T);
+    const size_t nblk = (cand + 255) / 256;
+    int* blk = nullptr;  // per-block counts, then per-block first slots
+    std::vector<int> base(nblk);
+    int64_t total = m->samples;
+    if (e == hipSuccess && nblk > 0) e = hipMalloc(&blk, nblk * sizeof(int));
+    if (e == hipSuccess && nblk > 0) {
+        hipLaunchKernelGGL(k_mlp_transition_count, dim3((unsigned)nblk), dim3(256), 0, m->stream, sd, nx, ny,
+                           (uint32_t)stride, (uint32_t)offset, blk);
         e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(base.data(), blk, nblk * sizeof(int), hipMemcpyDeviceToHost, m->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+        if (e == hipSuccess) {
+            // exclusive scan from the running count; bases past the capacity
+            // are clamped (those blocks store nothing)
+            for (size_t b = 0; b < nblk; ++b) {
+                const int cnt = base[b];
+                base[b] = (int)std::min<int64_t>(total, m->capacity);
+                total += cnt;
+            }
+            e = hipMemcpyAsync(blk, base.data(), nblk * sizeof(int), hipMemcpyHostToDevice, m->stream);
+        }
+        if (e == hipSuccess) {
+            PhysArgs pa{};
+            pa.nx = nx;
+            pa.ny = ny;
+            pa.omega = 1.0f / tau;
+            pa.gx = force_x;
+            pa.gy = force_y;
+            pa.scale = pa.inv_scale = 1.0f;
+            set_force_terms(pa);
+            hipLaunchKernelGGL(k_mlp_transition, dim3((unsigned)nblk), dim3(256), 0, m->stream, s0, s1, sd, pa,
+                               m->s_in, m->s_out, (uint32_t)stride, (uint32_t)offset, m->capacity, blk, m->X, m->T);
+            e = hipGetLastError();
+        }
     }
-    int total = m->samples;
-    if (e == hipSuccess) e = hipMemcpyAsync(&total, m->count, sizeof(int), hipMemcpyDeviceToHost, m->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
     (void)hipFree(s0);
     if (s1) (void)hipFree(s1);
     if (sd) (void)hipFree(sd);
+    if (blk) (void)hipFree(blk);
     if (e != hipSuccess) return lbm_fail(LBM_ERR_HIP, "transition samples: %s", hipGetErrorString(e));
-    total = std::min(total, m->capacity);
-    if (added) *added = total - m->samples;
-    m->samples = total;
+    total = std::min<int64_t>(total, m->capacity);
+    if (added) *added = (int)(total - m->samples);
+    m->samples = (int)total;
     return LBM_OK;
 }
 

@@ -304,6 +304,44 @@ def equilibrium(ux, uy):
     return f
 
 
+def test_transition_samples_and_training_are_reproducible():
+    """Samples are stored in candidate order (count pass, scan, ordered
+    writes), so two networks fed the same transitions train to the same
+    parameters bit for bit.  The sample count equals the number of candidate
+    cells (stride 3, offset 1) whose periodic 3x3 neighbourhood is fluid,
+    counted on the host, and a capacity below it keeps exactly `capacity`."""
+    print("case: reproducible transition samples and training", flush=True)
+    nf = 128
+    scene = W.periodic_disks(nf, nf, 0.08, 6, 10, seed=12)
+    ux, uy = vortex_field(nf, 0.05)
+    fine = L.Lattice(nf, nf, 0.52)
+    fine.write_materials(W.materials_from_solid(scene.solid))
+    fine.write_populations(equilibrium(ux, uy).astype(np.float32), 0)
+    fine.run(40)
+    s0, solid_c, _, _ = fine.restrict4(0.505)
+    fine.run(4)
+    s1, _, _, _ = fine.restrict4(0.505)
+    fine.close()
+    assert solid_c.any() and not solid_c.all()
+    near = np.zeros_like(solid_c)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            near |= np.roll(np.roll(solid_c, dy, 0), dx, 1)
+    expect = int(np.count_nonzero(~near.ravel()[1::3]))
+    nets = []
+    for cap in (40000, 40000, expect // 2):
+        net = L.Network(seed=4, capacity=cap)
+        net.set_scales(1.0, 1.0)
+        added = net.add_transition(s0, s1, solid_c, 0.505, stride=3, offset=1)
+        assert added == min(expect, cap) == net.sample_count()
+        nets.append(net)
+    losses = [net.train(steps=30, batch=512, lr=1e-3, seed=5) for net in nets[:2]]
+    assert losses[0] == losses[1]
+    assert np.array_equal(nets[0].params(), nets[1].params())
+    for net in nets:
+        net.close()
+
+
 def test_learned_coarse_beats_plain_coarse():
     """What the trained network gains.  Fine 256^2, tau_f = 0.52, wall-free
     periodic flow of decaying vortices (U = 0.05) restricted every 4 fine
