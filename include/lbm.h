@@ -349,7 +349,10 @@ int lbm_mlp_add_samples(lbm_mlp* m, const float* x, const float* t, int n);
  * t_a = clamp(d_a n_a / (n_a^2 + eps^2)) / out_scale for a = e, p_xx, p_xy (0 for
  * outputs 3..8), d_a the moments of the residual f_i(c + e_i, t+1) -
  * BGK_i(f(c, t)), eps^2 = 1e-2 (n_xx^2 + n_xy^2 + n_e^2 / 9): the
- * modulation that reproduces the residual.  Uses the current scales. */
+ * modulation that reproduces the residual.  Uses the current scales.
+ * Samples are appended in cell order (the same on every call with the same
+ * arguments); those beyond the capacity are dropped, and *added counts the
+ * stored ones. */
 int lbm_mlp_add_transition(lbm_mlp* m, const float* state_t, const float* state_t1, const uint8_t* solid, int nx,
                            int ny, float tau, float force_x, float force_y, int stride, int offset, int* added);
 /* Rescales the stored samples to unit root-mean-square (one factor for all
