@@ -84,13 +84,14 @@ def test_d3q19_vs_fp64_oracle(storage):
     lat.close()
 
 
-def test_d3q19_variants_and_graphs_bit_identical():
+@pytest.mark.parametrize("shape", [(72, 12, 10), (40, 70, 6)], ids=["72x12x10", "40x70x6"])
+def test_d3q19_variants_and_graphs_bit_identical(shape):
     """Every kernel shape, with 100-step graphs or direct launches, gives the
-    same populations at every slot a fluid cell can read: 72 x 12 x 10
-    (row segments cut by nx for every block size), spheres, force, 203
-    steps."""
-    nx, ny, nz = 72, 12, 10
-    print("case: d3q19 variants + graphs", flush=True)
+    same populations at every slot a fluid cell can read: 72 x 12 x 10 and
+    40 x 70 x 6 (row segments cut by nx for every block size; rows split
+    unevenly over the XCDs), spheres, force, 203 steps."""
+    nx, ny, nz = shape
+    print(f"case: d3q19 variants + graphs {shape}", flush=True)
     solid = spheres(nx, ny, nz, 5, 1, 3, seed=21)
     f0 = perturbed_rest((nz, ny, nx), 1e-2, seed=22).astype(np.float32)
     vis = visible_mask3(solid, D.E)
