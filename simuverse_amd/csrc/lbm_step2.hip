@@ -337,6 +337,8 @@ const BatchVariant kBatchVariants[] = {
     {256, 64, 0, kNtStore | kTwoStep, 8}, {256, 64, 0, kNtStore | kTwoStep | kXcdRemap, 16},
     {512, 64, 0, kNtStore | kTwoStep | kXcdRemap, 16}, {512, 128, 0, kNtStore | kTwoStep | kXcdRemap, 8},
     {256, 32, 0, kNtStore | kTwoStep | kXcdRemap, 16}, {1024, 64, 0, kNtStore | kTwoStep | kXcdRemap, 16},
+    {512, 64, 0, kNtStore | kTwoStep, 32}, {512, 64, 0, kNtStore | kTwoStep | kXcdRemap, 32},
+    {1024, 64, 0, kNtStore | kTwoStep | kXcdRemap, 32},
 };
 const int kNumBatchVariants = (int)(sizeof(kBatchVariants) / sizeof(kBatchVariants[0]));
 
@@ -392,6 +394,9 @@ static hipError_t launch_batch2_typed(int v, const BatchDomainDev* doms, const u
     case 17: LBM_BATCH2(512, 128, 8, kNtStore | kXcdRemap); break;
     case 18: LBM_BATCH2(256, 32, 16, kNtStore | kXcdRemap); break;
     case 19: LBM_BATCH2(1024, 64, 16, kNtStore | kXcdRemap); break;
+    case 20: LBM_BATCH2(512, 64, 32, kNtStore); break;
+    case 21: LBM_BATCH2(512, 64, 32, kNtStore | kXcdRemap); break;
+    case 22: LBM_BATCH2(1024, 64, 32, kNtStore | kXcdRemap); break;
     default: return hipErrorInvalidValue;
     }
 #undef LBM_BATCH2
