@@ -776,6 +776,7 @@ const StepVariant kStepVariants[] = {
     {512, 16, 64, kNtStore | kTwoStep | kXcdRemap},
     {1024, 16, 64, kNtStore | kTwoStep | kXcdRemap},
     {256, 8, 128, kNtStore | kTwoStep | kXcdRemap},
+    {512, 32, 64, kNtStore | kTwoStep | kXcdRemap},
 };
 const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
 

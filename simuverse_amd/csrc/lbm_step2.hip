@@ -306,6 +306,7 @@ static hipError_t dispatch_variant2(int v, const PhysArgs& a, const void* src, v
     case 30: return launch_phys_variant2<T, XEDGE, FORCE, 512, 64, 16, S | R>(a, src, dst, bits, words, s);
     case 31: return launch_phys_variant2<T, XEDGE, FORCE, 1024, 64, 16, S | R>(a, src, dst, bits, words, s);
     case 32: return launch_phys_variant2<T, XEDGE, FORCE, 256, 128, 8, S | R>(a, src, dst, bits, words, s);
+    case 33: return launch_phys_variant2<T, XEDGE, FORCE, 512, 64, 32, S | R>(a, src, dst, bits, words, s);
     default: return hipErrorInvalidValue;
     }
 }
