@@ -63,9 +63,8 @@ __device__ inline void two_step_tile(const PhysArgs& a, const T* __restrict__ sr
     // for a mask load.
     // A tile whose region and its gather neighbourhood lie inside the lattice
     // (no wrap, no inlet or outlet column) takes the same pulls with one
-    // pointer per cell and wave-uniform direction offsets (binary32; in
-    // binary16 the extra path costs more occupancy than it saves).
-    const bool inner = sizeof(T) == 4 && x0 >= 2 && x0 + BW + 1 <= nx - 1 && y0 >= 2 && y0 + BH + 1 <= ny - 1;
+    // pointer per cell and wave-uniform direction offsets.
+    const bool inner = x0 >= 2 && x0 + BW + 1 <= nx - 1 && y0 >= 2 && y0 + BH + 1 <= ny - 1;
     if (inner) {
 #pragma unroll
         for (int m = 0; m < kIter1; ++m) {
