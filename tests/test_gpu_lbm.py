@@ -54,6 +54,36 @@ def run_parity_pair(info, f0, f1, tau, steps, frames_api=False):
     return st, lat, cur
 
 
+def test_parity_and_physics_modes_agree_without_clamps():
+    """Cross-mode consistency of the two device modes on the case of the
+    oracle test of the same name (periodic all-bulk 64 x 48, tau 0.7, a
+    vortex with a density wave, 60 steps, every population inside the WGSL
+    clamps): the parity-mode mass over the physics-mode mass is the
+    truncated weights' closed form (1 - omega (1 - sum w))^t, and the
+    velocities agree to 2e-4 (the truncation's drift is 3.5e-5)."""
+    from test_oracle_wgsl import bulk_info, moments, smooth_state
+    print("case: parity vs physics mode, 64x48 all bulk", flush=True)
+    nx, ny, tau, steps = 64, 48, 0.7, 60
+    f = smooth_state(nx, ny).astype(np.float32)
+    par = L.Lattice(nx, ny, tau, mode=L.MODE_PARITY_WGSL, fluid_ty=1)
+    par.write_lattice_info(bulk_info(nx, ny))
+    par.write_populations(f, 0)
+    par.run(steps)
+    phy = L.Lattice(nx, ny, tau)
+    phy.write_populations(f, 0)
+    phy.run(steps)
+    r1, u1, v1 = moments(par.read_populations().astype(np.float64))
+    r2, u2, v2 = moments(phy.read_populations().astype(np.float64))
+    par.close()
+    phy.close()
+    omega = float(np.float32(1) / np.float32(tau))
+    decay = (1 - omega * (1 - O.wgsl_weights().astype(np.float64).sum())) ** steps
+    assert abs((r1 / r2).mean() / decay - 1) < 1e-6
+    assert np.abs(r1 / r2 / decay - 1).max() < 5e-6
+    du = np.sqrt((((u1 - u2) ** 2 + (v1 - v2) ** 2).sum()) / (u2**2 + v2**2).sum())
+    assert du < 2e-4
+
+
 def test_parity_c1_cylinder_1000_steps_bit_exact():
     """256x256 channel past a cylinder r=16 at (64,128), tau=0.6, inlet
     accelerate column vx=0.05, f0 = w + U(-1e-4, 1e-4), 1000 steps."""
