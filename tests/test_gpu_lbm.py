@@ -84,6 +84,47 @@ def test_parity_and_physics_modes_agree_without_clamps():
     assert du < 2e-4
 
 
+@pytest.mark.parametrize("nx, ny", [(3, 3), (4, 4), (5, 3), (7, 6), (16, 9)], ids=["3x3", "4x4", "5x3", "7x6", "16x9"])
+def test_tiny_periodic_lattices_against_oracles(nx, ny):
+    """Brute force on tiny periodic grids, where a two-step tile and its
+    one-cell ring wrap around the whole lattice: parity mode bit for bit
+    against the WGSL restatement (both buffers, every slot: no solids), and
+    physics mode with a body force (and a solid cell when the lattice has
+    room), with the default one-step shape and the 128 x 8 two-step shape,
+    against the binary64 oracle; after 1, 2, 7 and 100 steps."""
+    from test_oracle_wgsl import bulk_info
+    print(f"case: tiny lattice {nx}x{ny}", flush=True)
+    rng = np.random.default_rng(100 * nx + ny)
+    w = O.wgsl_weights()
+    for steps in (1, 2, 7, 100):
+        f0 = (w[:, None, None] + rng.uniform(-1e-3, 1e-3, (9, ny, nx))).astype(np.float32)
+        st, lat, _ = run_parity_pair(bulk_info(nx, ny), f0, f0.copy(), 0.6, steps)
+        for b in (0, 1):
+            assert np.array_equal(lat.read_populations(b), st.buf[b]), f"parity, {steps} steps, buffer {b}"
+        lat.close()
+    solid = np.zeros((ny, nx), bool)
+    if nx >= 4 and ny >= 4:
+        solid[ny // 2, nx // 2] = True
+    f0 = W.W_PHYS[:, None, None] * (1 + rng.uniform(-1e-3, 1e-3, (9, ny, nx)))
+    force = (1e-5, -5e-6)
+    for steps in (1, 2, 7, 100):
+        st = O.PhysState(O.phys_params(nx, ny, 0.7, force=force), solid, f0)
+        st.run(steps)
+        r0, u0, v0 = st.macro()
+        for variant in (None, 18):
+            lat = L.Lattice(nx, ny, 0.7, force=force)
+            lat.write_materials(W.materials_from_solid(solid))
+            lat.write_populations(f0.astype(np.float32), 0)
+            if variant is not None:
+                lat.set_variant(variant)
+            lat.run(steps)
+            r1, u1, v1 = lat.macro()
+            lat.close()
+            fl = ~solid
+            assert np.max(np.abs(r1[fl] - r0[fl])) < 1e-6, f"physics rho, {steps} steps, variant {variant}"
+            assert np.max(np.hypot(u1 - u0, v1 - v0)[fl]) < 1e-6, f"physics u, {steps} steps, variant {variant}"
+
+
 def test_parity_c1_cylinder_1000_steps_bit_exact():
     """256x256 channel past a cylinder r=16 at (64,128), tau=0.6, inlet
     accelerate column vx=0.05, f0 = w + U(-1e-4, 1e-4), 1000 steps."""
