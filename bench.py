@@ -8,11 +8,24 @@ One step = one fused collide-stream pass over the whole lattice.  Steps run
 as 100-step HIP-graph chunks after the tile shape has been autotuned.
 
 Prints one JSON line: MLUPS (all cells, solid included), the HBM roofline of
-the step kernel (72 algorithmic bytes per cell per step over the kernel's
-average duration from HIP events on its stream) and a CPU baseline: the
-binary32 restatement of the reference's WGSL (collide_stream + boundary
-passes, OpenMP over rows) timed on this host on a bounded sample of the same
-lattice without the body force, which the reference cannot express.
+the step kernel, the rho/u error against the binary64 oracle and a CPU
+baseline.
+
+  roofline   one launch of the step kernel must read and write the lattice
+             once, 72 B per cell in binary32 (36 B with binary16 deviations),
+             however many steps it performs; achieved = those bytes over the
+             average launch duration (HIP events on the library's stream
+             around the timed steps), frac = achieved / 8 TB/s <= 1.
+             effective_GBps credits 72 B per cell for every step, what
+             one-step launches would have to move: above the peak when a
+             launch performs two steps.
+  accuracy   the state after the timed run is stepped a further few steps on
+             the device and by the binary64 oracle (oracle/phys_oracle.c),
+             and rho and u are compared over the fluid cells.
+  cpu        the binary32 restatement of the reference's WGSL
+             (collide_stream + boundary passes, OpenMP over rows) timed on
+             this host on a bounded sample of the same lattice without the
+             body force, which the reference cannot express.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
   python bench.py --copy-calibration   # plain copy kernel, for byte counters
@@ -55,6 +68,52 @@ def traffic_from_profiles(workload: str, variant: int):
     if not m or not v or int(v.group(1)) != variant:
         return None
     return float(m.group(1))
+
+
+def roofline(bytes_per_cell: int, cells: int, step_s: float, per_launch: int, traffic):
+    """The step kernel against the HBM roofline.  A launch that performs
+    per_launch steps must still read and write every cell's populations
+    once (bytes_per_cell x cells); achieved is that over the average launch
+    duration, so frac <= 1.  effective_GBps credits bytes_per_cell for every
+    step instead."""
+    algo = bytes_per_cell * cells
+    launch_s = step_s * per_launch
+    achieved = algo / launch_s / 1e9
+    effective = algo / step_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "steps_per_launch": per_launch,
+            "algorithmic_bytes_per_launch": algo, "launch_us": round(launch_s * 1e6, 3),
+            "traffic_frac_of_peak": (round(traffic / launch_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None),
+            "effective_GBps": round(effective, 1), "effective_frac": round(effective / HBM_PEAK_GBS, 4)}
+
+
+def accuracy_vs_oracle(lat, solid, tau: float, force, steps: int = 20):
+    """rho and u of the device after `steps` more steps from its current
+    state, against the binary64 oracle run from the same populations: relative
+    L2 and largest absolute error over the fluid cells."""
+    from oracle import oracle as O
+
+    ny, nx = solid.shape
+    t0 = time.perf_counter()
+    f = lat.read_populations()
+    ps = O.PhysState(O.phys_params(nx, ny, tau, force=force), solid, f)
+    del f
+    lat.run(steps)
+    r_d, ux_d, uy_d = lat.macro()
+    ps.run(steps)
+    r_o, ux_o, uy_o = ps.macro()
+    del ps
+    fl = ~solid
+    du = np.hypot(ux_d - ux_o, uy_d - uy_o)[fl]
+    uo = np.hypot(ux_o, uy_o)[fl]
+    dr = (r_d - r_o)[fl]
+    out = {"vs": "binary64 oracle (oracle/phys_oracle.c)", "steps": steps, "from": "the state after the timed run",
+           "rho_l2_rel": float(np.linalg.norm(dr) / np.linalg.norm(r_o[fl])),
+           "rho_max_abs": float(np.max(np.abs(dr))),
+           "u_l2_rel": float(np.linalg.norm(du) / max(np.linalg.norm(uo), 1e-300)),
+           "u_max_abs": float(np.max(du)), "u_max": float(np.max(uo)),
+           "seconds": round(time.perf_counter() - t0, 1)}
+    return out
 
 
 def cpu_baseline(scene, budget_s: float = 12.0):
@@ -142,11 +201,17 @@ def run_batch(args):
     cells = b.cells()
     step_s = ms / 1e3 / K
     bpc = 36 if args.f16 else 72
-    achieved = bpc * cells / step_s / 1e9
     variant = b.variant()
     block, vec, tile_h, flags = L.batch_variant_shapes()[variant]
     per_launch = 2 if flags & 16 else 1
     workload = f"d2q9_batch512_{'f16dev' if args.f16 else 'f32'}"
+    roof = roofline(bpc, cells, step_s, per_launch, traffic_from_profiles(workload, variant))
+    # Domains whose state is no longer finite (BGK at tau near 1/2 with a
+    # fast inlet on a small domain): named, and their cells counted apart.
+    diverged = [k for k in range(len(doms)) if not np.isfinite(b.reduce(k)).all()]
+    finite_cells = sum(d.nx * d.ny for k, d in enumerate(doms) if k not in set(diverged))
+    log(f"batch: {K} steps {ms:.1f} ms, {cells / step_s / 1e6:.0f} MLUPS; {len(doms) - len(diverged)} of "
+        f"{len(doms)} domains finite after {args.warmup + K} steps; diverged: {diverged}")
     line = {
         "metric": METRIC,
         "value": round(cells / step_s / 1e6, 1),
@@ -163,10 +228,10 @@ def run_batch(args):
         "config": {"workload": workload, "domains": len(doms), "cells": int(cells), "variant": variant,
                    "batch_variant": {"kernel": "k_batch_step2" if per_launch == 2 else "k_batch_step",
                                      "block": block, "vec": vec, "tile_rows": tile_h, "flags": flags},
-                   "graph_chunk": args.graph_chunk, "seed": W.SEED},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(workload, variant),
-                     "steps_per_launch": per_launch, "algorithmic_bytes_per_launch": bpc * cells * per_launch},
+                   "graph_chunk": args.graph_chunk, "seed": W.SEED,
+                   "finite_domains": len(doms) - len(diverged), "diverged_domains": diverged,
+                   "finite_cells": int(finite_cells)},
+        "roofline": roof,
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -204,13 +269,13 @@ def run_d3q19(args):
     cells = n ** 3
     step_s = ms / 1e3 / K
     bpc = 76 if args.f16 else 152
-    achieved = bpc * cells / step_s / 1e9
+    workload = f"d3q19_{'f16dev' if args.f16 else 'f32'}_{n}cubed"
+    roof = roofline(bpc, cells, step_s, 1, traffic_from_profiles(workload, variant))
     r = lat.reduce()
-    log(f"{K} steps: {ms:.1f} ms, {step_s * 1e6:.2f} us/step, {cells / step_s / 1e6:.0f} MLUPS, {achieved:.0f} GB/s; "
+    log(f"{K} steps: {ms:.1f} ms, {step_s * 1e6:.2f} us/step, {cells / step_s / 1e6:.0f} MLUPS, {roof['achieved']:.0f} GB/s; "
         f"mass {r[0]:.9e} drag ({r[4]:.3e}, {r[5]:.3e}, {r[6]:.3e})")
     if not np.isfinite(r).all():
         raise SystemExit("non-finite state after the timed run")
-    workload = f"d3q19_{'f16dev' if args.f16 else 'f32'}_{n}cubed"
     block, cpt, flags = L.variant3_shapes()[variant]
     line = {
         "metric": METRIC,
@@ -230,146 +295,35 @@ def run_d3q19(args):
                    "seed": W.SEED, "variant": variant,
                    "tile_variant": {"kernel": "k3_step", "block": block, "cells_per_thread": cpt, "flags": flags},
                    "graph_chunk": args.graph_chunk},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(workload, variant),
-                     "steps_per_launch": 1, "algorithmic_bytes_per_launch": bpc * cells},
+        "roofline": roof,
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 
 
 def run_learned(args):
-    """Learned collision on a 4x coarser lattice against the fine solution.
-
-    Fine: n x n binary32, tau_f = 0.52 (nu = 1/150), periodic 4x4 array of
-    cylinders of radius n/64, started from a uniform flow U = 0.05 with a Guo
-    body force g_f holding it (Re = U D / nu = 0.05 * n/32 * 150, 1920 at
-    n = 8192).  Coarse: (n/4)^2, tau_c = 0.505 (nu_c = nu_f / 4, the same
-    flow in acoustic scaling), force 4 g_f, majority-solid restriction.
-
-    1. fine run of N_f steps (timed); every `pair_every` fine steps the state
-       and the one 4 fine steps later are restricted and their transition
-       goes into the network's sample store (inputs: non-equilibrium part of
-       the coarse populations; target: the residual of one coarse BGK step);
-    2. Adam training on those samples (binary32, device);
-    3. a validation gate: the gain on the correction (1, 1/2, 1/4 or 0, plain
-       BGK) with the smallest coarse velocity error at N_f / 2, the end of
-       the sampled half of the window;
-    4. coarse runs of N_f / 4 steps from the restricted fine initial state,
-       plain BGK and BGK + the gated network on the matrix cores (timed);
-    5. velocity relative L2 error of each coarse run against the restricted
-       fine velocity at N_f, and time-to-solution ratios.
-    """
-    from simuverse_amd import lbm as L
-    from simuverse_amd import workloads as W
+    """Learned collision on a 4x coarser lattice against the fine solution
+    (simuverse_amd/learned.py): fine n x n binary32 at tau 0.52 on a periodic
+    4x4 cylinder array (Re 1920 at n = 8192), samples from its 4x
+    restriction, device training, a validation gate against plain BGK, and
+    timed coarse runs at tau 0.505.  The line's value is the learned coarse
+    step's MLUPS; the config carries the errors and time-to-solution
+    ratios."""
+    from simuverse_amd import learned as LC
 
     n = args.n if args.n != 4096 else 8192
-    nc = n // 4
-    tau_f, tau_c, U = 0.52, 0.505, 0.05
-    gf = args.learned_force
-    gc = (4 * gf, 0.0)
-    Nf = args.learned_fine_steps
-    solid = W.cylinder_array_solid(n, 4, n // 64)
-    log(f"learned: fine {n}^2 tau {tau_f}, coarse {nc}^2 tau {tau_c}, {Nf} fine steps, "
-        f"solid {solid.mean():.4f}, Re {U * (n // 32) / ((tau_f - 0.5) / 3):.0f}")
-    fine = L.Lattice(n, n, tau_f, force=(gf, 0.0))
-    fine.write_materials(W.materials_from_solid(solid))
-    fine.write_populations(W.equilibrium_state(n, n, U, 0.0, solid).astype(np.float32), 0)
-    del solid
-    fine.autotune(steps=10)
-    # the timed fine solution: the whole N_f steps
-    fc0, solid_c, _, _ = fine.restrict4(tau_c, gc)
-    ms_fine = fine.run_timed(Nf)
-    _, _, uxr, uyr = fine.restrict4(tau_c, gc)
-    log(f"fine: {Nf} steps {ms_fine:.1f} ms ({n * n * Nf / ms_fine / 1e3:.0f} MLUPS)")
-
-    # training data from a second pass over the first half of the window
-    t0 = time.perf_counter()
-    net = L.Network(seed=7, capacity=args.learned_capacity)
-    fine2 = L.Lattice(n, n, tau_f, force=(gf, 0.0))
-    solid = W.cylinder_array_solid(n, 4, n // 64)
-    fine2.write_materials(W.materials_from_solid(solid))
-    fine2.write_populations(W.equilibrium_state(n, n, U, 0.0, solid).astype(np.float32), 0)
-    del solid
-    fine2.set_variant(fine.variant())
-    done, pairs = 0, 0
-    net.set_scales(1.0, 1.0)  # normalize_samples sets them from the samples
-    stride = max(1, (nc * nc * (Nf // 2 // args.pair_every)) // args.learned_capacity + 1)
-    # transitions from the second quarter of the window on (past the start-up
-    # transient) to its middle
-    for t in range(Nf // 8, Nf // 2, args.pair_every):
-        fine2.run(t - done)
-        done = t
-        a, _, _, _ = fine2.restrict4(tau_c, gc)
-        fine2.run(4)
-        done += 4
-        b, _, _, _ = fine2.restrict4(tau_c, gc)
-        net.add_transition(a, b, solid_c, tau_c, gc, stride=stride, offset=pairs % stride)
-        pairs += 1
-    # the restricted fine velocity at N_f / 2: the validation target
-    if done <= Nf // 2:
-        fine2.run(Nf // 2 - done)
-    _, _, uxh, uyh = fine2.restrict4(tau_c, gc)
-    fine2.close()
-    rx, rt = net.normalize_samples()
-    t_collect = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    loss = net.train(steps=args.learned_train_steps, batch=min(args.learned_batch, net.sample_count()), lr=2e-3,
-                     seed=3)
-    t_train = time.perf_counter() - t0
-    log(f"samples {net.sample_count()} from {pairs} transitions ({t_collect:.1f} s), "
-        f"train {args.learned_train_steps} steps ({t_train:.1f} s), loss {loss:.3e}")
-
-    def coarse(learned: bool, steps: int, ref_u):
-        lat = L.Lattice(nc, nc, tau_c, force=gc)
-        lat.write_materials(W.materials_from_solid(solid_c))
-        lat.write_populations(fc0, 0)
-        if learned:
-            lat.set_learned_collision(net)
-        else:
-            lat.autotune(steps=10)
-            lat.write_populations(fc0, 0)
-        ms = lat.run_timed(steps)
-        _, ux, uy = lat.macro()
-        u, finite = lat.health()
-        fl = ~solid_c
-        ur, vr = ref_u
-        ref = np.hypot(ur, vr)[fl]
-        err = float(np.linalg.norm(np.hypot(ux - ur, uy - vr)[fl]) / np.linalg.norm(ref)) if finite else float("inf")
-        lat.close()
-        return ms, err
-
-    # Validation gate: the correction's gain (a factor on the output scale)
-    # is chosen on the first half of the window, coarse runs to N_f / 2
-    # against the restricted fine velocity there; 0 is plain BGK.  The
-    # reported errors are at N_f, which no sample or gate run has seen.
-    s_in, s_out = net.scales()
-    gate = {}
-    for gain in (1.0, 0.5, 0.25, 0.0):
-        if gain > 0.0:
-            net.set_scales(s_in, s_out * gain)
-        gate[gain] = coarse(gain > 0.0, Nf // 8, (uxh, uyh))[1]
-    gain = min(gate, key=lambda k: (gate[k], -k))
-    log("validation gate (vel L2 at N_f/2 by gain): " + ", ".join(f"{k:g}: {v:.4e}" for k, v in gate.items())
-        + f"; chosen gain {gain:g}")
-
-    ms_plain, e_plain = coarse(False, Nf // 4, (uxr, uyr))
-    # the learned kernel at the chosen gain; at gain 0 the gate falls back to
-    # plain BGK and the learned kernel is timed at gain 1 for reference
-    net.set_scales(s_in, s_out * (gain if gain > 0.0 else 1.0))
-    ms_learn, e_learn = coarse(True, Nf // 4, (uxr, uyr))
-    e_gated, ms_gated = (e_learn, ms_learn) if gain > 0.0 else (e_plain, ms_plain)
-    log(f"coarse plain: {ms_plain:.1f} ms, vel L2 {e_plain:.4e}; learned (gain {gain if gain > 0 else 1:g}): {ms_learn:.1f} ms, "
-        f"vel L2 {e_learn:.4e}")
-    K = Nf // 4
+    r = LC.run(n=n, fine_steps=args.learned_fine_steps, force=args.learned_force, pair_every=args.pair_every,
+               capacity=args.learned_capacity, train_steps=args.learned_train_steps, batch=args.learned_batch,
+               log=log)
+    nc, K = n // 4, r.coarse_steps
     line = {
         "metric": METRIC,
-        "value": round(nc * nc * K / ms_learn / 1e3, 1),
+        "value": round(nc * nc * K / r.learned_ms / 1e3, 1),
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": K,
         "warmup": 0,
-        "ms_per_step": round(ms_learn / K, 6),
+        "ms_per_step": round(r.learned_ms / K, 6),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -377,24 +331,23 @@ def run_learned(args):
         "data": "synthetic",
         "config": {
             "workload": f"d2q9_learned_{n}_to_{nc}",
-            "fine": {"n": n, "tau": tau_f, "steps": Nf, "ms": round(ms_fine, 3)},
-            "coarse": {"n": nc, "tau": tau_c, "steps": K},
-            "force_fine": gf, "U0": U, "cylinders": "4x4, r = n/64",
-            "training": {"pairs": pairs, "samples": net.sample_count(), "steps": args.learned_train_steps,
-                         "batch": min(args.learned_batch, net.sample_count()), "final_loss": loss,
-                         "collect_s": round(t_collect, 2), "train_s": round(t_train, 2)},
-            "validation_gate": {"vel_l2_at_half_window_by_gain": {f"{k:g}": v for k, v in gate.items()},
-                                "gain": gain, "vel_l2": e_gated, "coarse_ms": round(ms_gated, 3),
-                                "speedup_vs_fine": round(ms_fine / ms_gated, 2)},
-            "vel_l2_vs_restricted_fine": {"plain": e_plain, "learned": e_learn},
-            "coarse_ms": {"plain": round(ms_plain, 3), "learned": round(ms_learn, 3)},
-            "time_to_solution_speedup": {"learned_vs_fine": round(ms_fine / ms_learn, 2),
+            "fine": {"n": n, "tau": LC.TAU_FINE, "steps": r.fine_steps, "ms": round(r.fine_ms, 3)},
+            "coarse": {"n": nc, "tau": LC.TAU_COARSE, "steps": K},
+            "force_fine": args.learned_force, "U0": LC.U0, "cylinders": "4x4, r = n/64",
+            "reynolds": round(r.reynolds, 1),
+            "training": {"pairs": r.pairs, "samples": r.samples, "steps": args.learned_train_steps,
+                         "batch": min(args.learned_batch, r.samples), "final_loss": r.final_loss,
+                         "collect_s": round(r.collect_s, 2), "train_s": round(r.train_s, 2)},
+            "validation_gate": {"vel_l2_at_half_window_by_gain": {f"{k:g}": v for k, v in r.gate_errors.items()},
+                                "gain": r.gain, "vel_l2": r.gated_err, "coarse_ms": round(r.gated_ms, 3),
+                                "speedup_vs_fine": round(r.fine_ms / r.gated_ms, 2)},
+            "vel_l2_vs_restricted_fine": {"plain": r.plain_err, "learned": r.learned_err},
+            "coarse_ms": {"plain": round(r.plain_ms, 3), "learned": round(r.learned_ms, 3)},
+            "time_to_solution_speedup": {"learned_vs_fine": round(r.fine_ms / r.learned_ms, 2),
                                          "learned_incl_training_vs_fine":
-                                             round(ms_fine / (ms_learn + 1e3 * (t_collect + t_train)), 3)},
+                                             round(r.fine_ms / (r.learned_ms + 1e3 * (r.collect_s + r.train_s)), 3)},
         },
-        "roofline": {"bound": "hbm", "achieved": round(72 * nc * nc * K / ms_learn / 1e6, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(72 * nc * nc * K / ms_learn / 1e6 / HBM_PEAK_GBS, 4), "traffic": None},
+        "roofline": roofline(72, nc * nc, r.learned_ms / 1e3 / K, 1, None),
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -407,6 +360,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=20,
+                    help="steps after the timed run checked against the binary64 oracle (0: skip)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--variant", type=int, default=-1)
     ap.add_argument("--copy-calibration", action="store_true")
@@ -476,21 +431,27 @@ def main():
     mlups = cells * K / sec / 1e6
     step_us = ms * 1e3 / K
     bytes_per_cell = 36 if args.f16 else BYTES_PER_CELL_F32
-    achieved = bytes_per_cell * cells / (step_us * 1e-6) / 1e9
     workload = f"d2q9_phys_{'f16dev' if args.f16 else 'f32'}_{n}x{n}"
+    traffic = traffic_from_profiles(workload, variant)
+    per_launch = describe_variant(L.variant_shapes()[variant])["steps_per_launch"]
+    roof = roofline(bytes_per_cell, cells, step_us * 1e-6, per_launch, traffic)
     r = lat.reduce()
-    log(f"{K} steps: {ms:.1f} ms events, {wall*1e3:.1f} ms wall, {step_us:.2f} us/step, {mlups:.0f} MLUPS, "
-        f"{achieved:.0f} GB/s; mass {r[0]:.9e} drag ({r[3]:.3e}, {r[4]:.3e})")
+    log(f"{K} steps: {ms:.1f} ms events, {wall*1e3:.1f} ms wall, {step_us:.2f} us/step, {mlups:.0f} MLUPS; "
+        f"launch {roof['launch_us']:.1f} us, {roof['achieved']:.0f} GB/s of the bytes a launch must move "
+        f"(frac {roof['frac']:.3f}), effective {roof['effective_GBps']:.0f} GB/s; "
+        f"mass {r[0]:.9e} drag ({r[3]:.3e}, {r[4]:.3e})")
     if not np.isfinite(r).all():
         raise SystemExit("non-finite state after the timed run")
+
+    acc = None
+    if args.accuracy_steps > 0:
+        acc = accuracy_vs_oracle(lat, scene.solid, tau, (g, 0.0), args.accuracy_steps)
+        log(f"accuracy vs binary64 oracle: {acc}")
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene)
         log(f"cpu baseline: {cpu}")
-
-    traffic = traffic_from_profiles(workload, variant)
-    per_launch = describe_variant(L.variant_shapes()[variant])["steps_per_launch"]
     line = {
         "metric": METRIC,
         "value": round(mlups, 1),
@@ -519,16 +480,8 @@ def main():
             "graph_chunk": args.graph_chunk,
             "wall_ms": round(wall * 1e3, 3),
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "steps_per_launch": per_launch,
-            "algorithmic_bytes_per_launch": bytes_per_cell * cells * per_launch,
-        },
+        "roofline": roof,
+        "accuracy": acc,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
