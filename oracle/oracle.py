@@ -7,6 +7,8 @@ simuverse_amd/ never imports it.
   WGSL restatement (binary32)   wgsl_restate.c, follows the reference shaders
   physics oracle (binary64)     phys_oracle.c
   MLP oracle (binary64)         mlp_oracle.c
+  learned step (binary64)       learned_oracle.c: the learned collision step,
+                                its training targets, the 4x restriction
 """
 from __future__ import annotations
 
@@ -71,6 +73,12 @@ def lib() -> C.CDLL:
             "mlp_forward": (None, [P, C.c_int, P, P]),
             "mlp_loss_grad": (C.c_double, [P, C.c_int, P, P, P]),
             "mlp_adam": (None, [P, P, P, P, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),
+            "lo_modulation_bound": (C.c_double, [C.c_double]),
+            "lo_step": (C.c_int, [C.POINTER(PoParams), P, P, P, P, C.c_double, C.c_double, P]),
+            "lo_run": (C.c_int, [C.POINTER(PoParams), P, P, P, P, C.c_double, C.c_double, C.c_int]),
+            "lo_transition": (C.c_int, [C.POINTER(PoParams), P, P, P, C.c_double, C.c_double, C.c_int, C.c_int,
+                                        C.c_int, P, P]),
+            "lo_restrict4": (None, [C.POINTER(PoParams), P, P, C.c_double, C.c_double, C.c_double, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -233,3 +241,80 @@ def mlp_adam(params: np.ndarray, grad: np.ndarray, m: np.ndarray, v: np.ndarray,
     for a in (params, grad, m, v):
         assert a.dtype == np.float64 and a.flags.c_contiguous
     lib().mlp_adam(_p(params), _p(grad), _p(m), _p(v), step, lr, beta1, beta2, eps)
+
+
+# ---------------------------------------------------------------------------
+# Learned collision step, its training targets, the 4x restriction (binary64)
+# ---------------------------------------------------------------------------
+
+
+def modulation_bound(omega: float) -> float:
+    return lib().lo_modulation_bound(omega)
+
+
+def _net64(net_params: np.ndarray) -> np.ndarray:
+    p = np.ascontiguousarray(net_params, dtype=np.float64)
+    assert p.size == mlp_param_count()
+    return p
+
+
+def learned_step(params: PoParams, solid: np.ndarray, post: np.ndarray, net_params: np.ndarray, s_in: float,
+                 s_out: float) -> tuple[np.ndarray, np.ndarray]:
+    """One learned step of post-collision populations (9, ny, nx): the next
+    post-collision state and the map of corrected cells."""
+    solid = np.ascontiguousarray(solid, dtype=np.uint8)
+    a = np.ascontiguousarray(post, dtype=np.float64).reshape(9, params.ny, params.nx)
+    out = np.empty_like(a)
+    learned = np.zeros((params.ny, params.nx), np.uint8)
+    rc = lib().lo_step(C.byref(params), _p(solid), _p(a), _p(out), _p(_net64(net_params)), s_in, s_out,
+                       _p(learned))
+    assert rc == 0, "oracle out of memory"
+    return out, learned.astype(bool)
+
+
+def learned_run(params: PoParams, solid: np.ndarray, post: np.ndarray, net_params: np.ndarray, s_in: float,
+                s_out: float, nsteps: int) -> np.ndarray:
+    solid = np.ascontiguousarray(solid, dtype=np.uint8)
+    a = np.ascontiguousarray(post, dtype=np.float64).reshape(9, params.ny, params.nx).copy()
+    a[:, solid.astype(bool)] = 0.0
+    b = np.empty_like(a)
+    rc = lib().lo_run(C.byref(params), _p(solid), _p(a), _p(b), _p(_net64(net_params)), s_in, s_out, nsteps)
+    assert rc == 0, "oracle out of memory"
+    return a
+
+
+def transition_samples(s0: np.ndarray, s1: np.ndarray, solid: np.ndarray, tau: float,
+                       force: tuple[float, float] = (0.0, 0.0), s_in: float = 1.0, s_out: float = 1.0,
+                       stride: int = 1, offset: int = 0, max_samples: int | None = None):
+    """Training samples (X, T), each (count, 9), of the transition s0 -> s1
+    (pre-collision states, dense (9, ny, nx)) of a periodic lattice."""
+    s0 = np.ascontiguousarray(s0, dtype=np.float64)
+    s1 = np.ascontiguousarray(s1, dtype=np.float64)
+    solid = np.ascontiguousarray(solid, dtype=np.uint8)
+    ny, nx = solid.shape
+    if max_samples is None:
+        max_samples = (nx * ny - offset + stride - 1) // stride
+    X = np.zeros((max_samples, 9))
+    T = np.zeros((max_samples, 9))
+    p = phys_params(nx, ny, tau, force=force)
+    k = lib().lo_transition(C.byref(p), _p(solid), _p(s0), _p(s1), s_in, s_out, stride, offset, max_samples,
+                            _p(X), _p(T))
+    return X[:k], T[:k]
+
+
+def restrict4_state(params: PoParams, solid: np.ndarray, post: np.ndarray, tau_c: float,
+                    force_c: tuple[float, float] = (0.0, 0.0)):
+    """4x restriction of post-collision populations (9, ny, nx): coarse
+    pre-collision populations (9, ny/4, nx/4), the coarse solid map and the
+    coarse velocity."""
+    solid = np.ascontiguousarray(solid, dtype=np.uint8)
+    post = np.ascontiguousarray(post, dtype=np.float64)
+    ny, nx = solid.shape
+    cy, cx = ny // 4, nx // 4
+    fc = np.zeros((9, cy, cx))
+    sc = np.zeros((cy, cx), np.uint8)
+    ux = np.zeros((cy, cx))
+    uy = np.zeros((cy, cx))
+    lib().lo_restrict4(C.byref(params), _p(solid), _p(post), tau_c, force_c[0], force_c[1], _p(fc), _p(sc), _p(ux),
+                       _p(uy))
+    return fc, sc.astype(bool), ux, uy

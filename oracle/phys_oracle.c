@@ -35,13 +35,7 @@
 #include <stdint.h>
 #include <string.h>
 
-typedef struct {
-    int32_t nx, ny;
-    double tau;
-    int32_t x_edge; /* 0 periodic, 1 Zou-He inlet at x=0 + zero-gradient outlet */
-    double inlet_ux;
-    double gx, gy;
-} po_params;
+#include "phys_oracle.h"
 
 static const int PO_EX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 static const int PO_EY[9] = {0, 0, -1, 0, 1, -1, -1, 1, 1};
