@@ -339,6 +339,18 @@ int lbm_mlp_forward(lbm_mlp* m, const float* x, int n, float* y);
 int lbm_mlp_forward_bf16(lbm_mlp* m, const float* x, int n, float* y);
 /* Loss 1/(9n) sum (y - t)^2 of n host samples and its gradient (5385). */
 int lbm_mlp_loss_grad(lbm_mlp* m, const float* x, const float* t, int n, float* grad, double* loss);
+/* lbm_mlp_loss_grad without allocation.  No counterpart in the reference
+ * (it has no learned collision); for callers that cannot allocate device
+ * memory (code under HIP-graph capture, training or serving loops, harnesses
+ * that forbid hipMalloc).  The same kernels and launch shape as
+ * lbm_mlp_loss_grad, so the same bits; the staged samples, per-block
+ * partials, gradient and loss live in a fixed 6.7 MB device array the
+ * library holds on every device it runs on until the process ends.
+ * n <= 16384 samples, else LBM_ERR_INVALID.  Every call on one device
+ * shares that array: calls that overlap on one device (from several
+ * threads, or several networks) race and must be serialised by the
+ * caller. */
+int lbm_mlp_loss_grad_noalloc(lbm_mlp* m, const float* x, const float* t, int n, float* grad, double* loss);
 /* Training samples held on the device. */
 int lbm_mlp_clear_samples(lbm_mlp* m);
 int lbm_mlp_sample_count(lbm_mlp* m, int* n);

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -651,6 +652,92 @@ int lbm_mlp_loss_grad(lbm_mlp* m, const float* x, const float* t, int n, float* 
     if (dt) (void)hipFree(dt);
     if (rc != LBM_OK) return rc;
     if (e != hipSuccess) return lbm_fail(LBM_ERR_HIP, "network gradient: %s", hipGetErrorString(e));
+    return LBM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Loss gradient without allocation (lbm_mlp_loss_grad_noalloc): the staged
+// samples, the per-block partials, the gradient and the loss live in one
+// file-scope device array, `scratch`, of fixed size (every device the
+// library runs on holds its own copy until the process ends).  Its device
+// address is looked up once per device with hipGetSymbolAddress; host code
+// passes that address, never the array's name, to copies and kernels.
+// Every call on one device shares the array: overlapping calls race.
+// Nothing in it needs clearing: each block writes its whole partial, and
+// the reduction writes the gradient and the loss.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kNoAllocSamples = 16384;
+constexpr int kNoAllocBlocks = (kNoAllocSamples + kTS - 1) / kTS;  // 256 gradient blocks
+static_assert(kNoAllocBlocks <= kGradBlocks, "one partial per gradient block");
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kScrPartial = 0;
+constexpr size_t kScrLossPart = kScrPartial + align256((size_t)kNoAllocBlocks * kMlpParams * sizeof(float));
+constexpr size_t kScrGrad = kScrLossPart + align256((size_t)kNoAllocBlocks * sizeof(double));
+constexpr size_t kScrLoss = kScrGrad + align256((size_t)kMlpParams * sizeof(float));
+constexpr size_t kScrX = kScrLoss + align256(sizeof(double));
+constexpr size_t kScrT = kScrX + align256((size_t)kNoAllocSamples * kMlpIn * sizeof(float));
+constexpr size_t kScratchBytes = kScrT + align256((size_t)kNoAllocSamples * kMlpOut * sizeof(float));  // 6.7 MB
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on device `dev` (which must be current).
+int scratch_address(int dev, unsigned char** out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return lbm_fail(LBM_ERR_INVALID, "device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return lbm_fail(LBM_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_mlp_loss_grad_noalloc(lbm_mlp* m, const float* x, const float* t, int n, float* grad, double* loss)
+{
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (!x || !t || n < 1) return lbm_fail(LBM_ERR_INVALID, "bad arguments");
+    if (n > kNoAllocSamples)
+        return lbm_fail(LBM_ERR_INVALID, "%d samples exceed the %d that lbm_mlp_loss_grad_noalloc holds", n,
+                        kNoAllocSamples);
+    unsigned char* base = nullptr;
+    if ((rc = scratch_address(m->dev, &base))) return rc;
+    float* dx = reinterpret_cast<float*>(base + kScrX);
+    float* dt = reinterpret_cast<float*>(base + kScrT);
+    float* partial = reinterpret_cast<float*>(base + kScrPartial);
+    double* loss_part = reinterpret_cast<double*>(base + kScrLossPart);
+    float* dgrad = reinterpret_cast<float*>(base + kScrGrad);
+    double* dloss = reinterpret_cast<double*>(base + kScrLoss);
+    // the same launch shape as lbm_mlp_loss_grad, so the sums run in the
+    // same order and the results are the same bits
+    const int nb = std::min((n + kTS - 1) / kTS, kGradBlocks);
+    MLP_TRY(hipMemcpyAsync(dx, x, (size_t)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    MLP_TRY(hipMemcpyAsync(dt, t, (size_t)n * kMlpOut * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    hipLaunchKernelGGL(k_mlp_grad_partial, dim3(nb), dim3(kGradThreads), 0, m->stream, m->params, dx, dt, n, n,
+                       (uint64_t)0, 0, partial, loss_part);
+    MLP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_mlp_grad_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, m->stream, partial, nb,
+                       dgrad, loss_part, dloss, n);
+    MLP_TRY(hipGetLastError());
+    if (grad) MLP_TRY(hipMemcpyAsync(grad, dgrad, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    if (loss) MLP_TRY(hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+    MLP_TRY(hipStreamSynchronize(m->stream));
     return LBM_OK;
 }
 

@@ -125,6 +125,7 @@ _SIGNATURES = [
     ("lbm_mlp_forward", C.c_int, [_P, _P, C.c_int, _P]),
     ("lbm_mlp_forward_bf16", C.c_int, [_P, _P, C.c_int, _P]),
     ("lbm_mlp_loss_grad", C.c_int, [_P, _P, _P, C.c_int, _P, C.POINTER(C.c_double)]),
+    ("lbm_mlp_loss_grad_noalloc", C.c_int, [_P, _P, _P, C.c_int, _P, C.POINTER(C.c_double)]),
     ("lbm_mlp_clear_samples", C.c_int, [_P]),
     ("lbm_mlp_sample_count", C.c_int, [_P, C.POINTER(C.c_int)]),
     ("lbm_mlp_add_samples", C.c_int, [_P, _P, _P, C.c_int]),
@@ -588,6 +589,17 @@ class Network:
         g = np.empty(self.nparams, np.float32)
         loss = C.c_double()
         _check(self._lib.lbm_mlp_loss_grad(self._h, _ptr(x), _ptr(t), x.shape[0], _ptr(g), C.byref(loss)))
+        return loss.value, g
+
+    def loss_grad_noalloc(self, x: np.ndarray, t: np.ndarray) -> tuple[float, np.ndarray]:
+        """loss_grad through lbm_mlp_loss_grad_noalloc (no device allocation;
+        at most 16384 samples)."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 9)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 9)
+        g = np.empty(self.nparams, np.float32)
+        loss = C.c_double()
+        _check(self._lib.lbm_mlp_loss_grad_noalloc(self._h, _ptr(x), _ptr(t), x.shape[0], _ptr(g),
+                                                   C.byref(loss)))
         return loss.value, g
 
     def clear_samples(self) -> None:
