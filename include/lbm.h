@@ -177,10 +177,13 @@ int lbm_write_populations(lbm_handle* h, int buf, const float* f);
 int lbm_read_populations(lbm_handle* h, int buf, float* f);
 
 /* Checkpoint / resume (no reference counterpart: the reference only
- * re-runs init, fluid_simulator.rs:210-215).  The state is the device
- * image of the current population buffer, lbm_state_size() bytes; loading
- * it into a handle with the same descriptor and lattice info and stepping
- * reproduces an uninterrupted run bit for bit. */
+ * re-runs init, fluid_simulator.rs:210-215).  The state is a 64-byte
+ * header (ABI version, lattice size, mode, storage, layout, binary16 scale,
+ * strides) followed by the device image of the current population buffer,
+ * lbm_state_size() bytes in all; loading it into a handle with the same
+ * descriptor and lattice info and stepping reproduces an uninterrupted run
+ * bit for bit.  A state whose header does not match the handle is rejected
+ * with LBM_ERR_INVALID. */
 int lbm_state_size(lbm_handle* h, size_t* bytes);
 int lbm_save_state(lbm_handle* h, void* out, size_t bytes);
 int lbm_load_state(lbm_handle* h, const void* in, size_t bytes);
@@ -312,12 +315,14 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* best_variant, double* ms_pe
  * x_i = (f_i - feq_i) / (w_i * in_scale), to y; its outputs y_0..y_2
  * modulate the non-equilibrium parts n_a of the second-order non-conserved
  * moments a = e, p_xx, p_xy (orthogonal moment basis) by alpha_a * n_a,
- * alpha_a = clamp(out_scale * y_a, -(2 - omega)/2, (2 - omega)/2) (a bound
- * that keeps the relaxation stable and adds no viscosity on average),
+ * alpha_a = clamp(out_scale * y_a, -b, b), b = min((2 - omega)/2, omega/2)
+ * (a bound that keeps every relaxation factor 1 - omega + alpha_a inside
+ * (-1, 1) for any tau > 1/2 and adds no viscosity on average),
  * mapped back to populations (no mass, no
  * momentum, no ghost moments) and added to the BGK post-collision
  * populations of cells whose 3x3 neighbourhood is fluid (the cells the
- * network is trained on).  Outputs 3..8 are unused (trained to zero).
+ * network is trained on; with LBM_X_INLET_OUTLET not the inlet and outlet
+ * columns either).  Outputs 3..8 are unused (trained to zero).
  * Parameters: one vector of 5385 floats,
  * W1[64][9] b1[64] W2[64][64] b2[64] W3[9][64] b3[9] (row = output unit).
  * Training (forward, backward, Adam) runs on the device in binary32. */

@@ -78,6 +78,7 @@ struct lbm_handle {
     uint32_t* bits = nullptr;  // packed solid mask (physics mode)
     uint32_t words = 0;        // mask words per row
     float* tmp3 = nullptr;   // lbm_macro output planes
+    uint16_t* texels = nullptr;  // rgba16float staging of the macro / curl readbacks
     double* partial = nullptr;
     double* red = nullptr;
     int cur = 0;
@@ -148,15 +149,15 @@ hipError_t enqueue_step(lbm_handle* h, int swap, bool export_macro)
     const void* src = h->buf[swap];
     void* dst = h->buf[swap ^ 1];
     if (h->physics() && h->mlp) {
-        if (export_macro) return hipErrorNotSupported;
         float s_in, s_out;
         lbm_mlp_scales(h->mlp, &s_in, &s_out);
+        float* macro = export_macro ? h->macro : nullptr;
         if (h->mlp_precision == LBM_MLP_BF16)
             return launch_phys_step_learned_mfma(h->f16, h->d.x_edge, h->force(), h->phys(), src, dst, h->bits,
                                                  h->words, lbm_mlp_device_packed(h->mlp),
-                                                 lbm_mlp_device_params(h->mlp), s_in, s_out, h->stream);
+                                                 lbm_mlp_device_params(h->mlp), s_in, s_out, macro, h->stream);
         return launch_phys_step_learned(h->f16, h->d.x_edge, h->force(), h->phys(), src, dst, h->bits, h->words,
-                                        lbm_mlp_device_params(h->mlp), s_in, s_out, h->stream);
+                                        lbm_mlp_device_params(h->mlp), s_in, s_out, macro, h->stream);
     }
     if (h->physics()) {
         const int v = step_variant_two_step(h->variant) ? kSingleStepFallback : h->variant;
@@ -317,13 +318,42 @@ void free_handle(lbm_handle* h)
 {
     drop_graphs(h);
     for (void* p : {h->buf[0], h->buf[1], (void*)h->mat, (void*)h->biter, (void*)h->fvel, (void*)h->macro,
-                    (void*)h->tmp3, (void*)h->partial, (void*)h->red, (void*)h->bits, (void*)h->health,
-                    h->backup})
+                    (void*)h->tmp3, (void*)h->texels, (void*)h->partial, (void*)h->red, (void*)h->bits,
+                    (void*)h->health, h->backup})
         if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
+}
+
+// Header of a saved state: what must match for the device image behind it
+// to mean the same populations in another handle.
+struct StateHeader {
+    char magic[8];  // "LBMSTATE"
+    int32_t abi;
+    int32_t nx, ny;
+    int32_t mode, storage, layout;
+    int32_t dev_shift;  // effective binary16 scale exponent (0 for binary32)
+    uint32_t plane, rs;
+    int32_t reserved[5];
+};
+static_assert(sizeof(StateHeader) == 64, "64-byte state header");
+
+StateHeader state_header(const lbm_handle* h)
+{
+    StateHeader hd{};
+    std::memcpy(hd.magic, "LBMSTATE", 8);
+    hd.abi = LBM_ABI_VERSION;
+    hd.nx = h->d.nx;
+    hd.ny = h->d.ny;
+    hd.mode = h->d.mode;
+    hd.storage = h->d.storage;
+    hd.layout = h->physics() ? h->d.layout : LBM_LAYOUT_PLANES;
+    hd.dev_shift = h->f16 ? (int32_t)std::lround(std::log2(h->scale)) : 0;
+    hd.plane = h->plane;
+    hd.rs = h->rs;
+    return hd;
 }
 
 int upload_info(lbm_handle* h, const uint8_t* m, const int32_t* bi, const float2* v, size_t offset, size_t count)
@@ -623,18 +653,23 @@ int lbm_read_macro(lbm_handle* h, float* rho, float* ux, float* uy)
     return LBM_OK;
 }
 
+// The texel staging buffer, allocated on first use and kept with the handle
+// (the readbacks run every frame).
+static int ensure_texels(lbm_handle* h)
+{
+    if (!h->texels) HIP_TRY(hipMalloc(&h->texels, (size_t)h->n * 8));
+    return LBM_OK;
+}
+
 int lbm_read_macro_rgba16f(lbm_handle* h, uint16_t* rgba)
 {
     int rc = check_handle(h);
     if (rc) return rc;
     if (!rgba) return fail(LBM_ERR_INVALID, "null output");
-    uint16_t* tmp = nullptr;
-    HIP_TRY(hipMalloc(&tmp, (size_t)h->n * 8));
-    hipError_t e = launch_pack_rgba16f(h->macro, h->n, tmp, h->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(rgba, tmp, (size_t)h->n * 8, hipMemcpyDeviceToHost, h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    (void)hipFree(tmp);
-    if (e != hipSuccess) return fail(LBM_ERR_HIP, "macro pack: %s", hipGetErrorString(e));
+    if ((rc = ensure_texels(h))) return rc;
+    HIP_TRY(launch_pack_rgba16f(h->macro, h->n, h->texels, h->stream));
+    HIP_TRY(hipMemcpyAsync(rgba, h->texels, (size_t)h->n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
     return LBM_OK;
 }
 
@@ -643,13 +678,10 @@ int lbm_read_curl_rgba16f(lbm_handle* h, uint16_t* rgba)
     int rc = check_handle(h);
     if (rc) return rc;
     if (!rgba) return fail(LBM_ERR_INVALID, "null output");
-    uint16_t* tmp = nullptr;
-    HIP_TRY(hipMalloc(&tmp, (size_t)h->n * 8));
-    hipError_t e = launch_curl_rgba16f(h->macro, h->d.nx, h->d.ny, tmp, h->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(rgba, tmp, (size_t)h->n * 8, hipMemcpyDeviceToHost, h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    (void)hipFree(tmp);
-    if (e != hipSuccess) return fail(LBM_ERR_HIP, "curl: %s", hipGetErrorString(e));
+    if ((rc = ensure_texels(h))) return rc;
+    HIP_TRY(launch_curl_rgba16f(h->macro, h->d.nx, h->d.ny, h->texels, h->stream));
+    HIP_TRY(hipMemcpyAsync(rgba, h->texels, (size_t)h->n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
     return LBM_OK;
 }
 
@@ -794,7 +826,7 @@ int lbm_autotune(lbm_handle* h, int steps, int* best_variant, double* ms_per_ste
 int lbm_state_size(lbm_handle* h, size_t* bytes)
 {
     if (!h || !bytes) return fail(LBM_ERR_INVALID, "null argument");
-    *bytes = h->pop_bytes();
+    *bytes = sizeof(StateHeader) + h->pop_bytes();
     return LBM_OK;
 }
 
@@ -802,9 +834,12 @@ int lbm_save_state(lbm_handle* h, void* out, size_t bytes)
 {
     int rc = check_handle(h);
     if (rc) return rc;
-    if (!out || bytes != h->pop_bytes())
-        return fail(LBM_ERR_INVALID, "state buffer must be %zu bytes", h->pop_bytes());
-    HIP_TRY(hipMemcpyAsync(out, h->buf[h->cur], bytes, hipMemcpyDeviceToHost, h->stream));
+    if (!out || bytes != sizeof(StateHeader) + h->pop_bytes())
+        return fail(LBM_ERR_INVALID, "state buffer must be %zu bytes", sizeof(StateHeader) + h->pop_bytes());
+    const StateHeader hd = state_header(h);
+    std::memcpy(out, &hd, sizeof hd);
+    HIP_TRY(hipMemcpyAsync(static_cast<char*>(out) + sizeof hd, h->buf[h->cur], h->pop_bytes(),
+                           hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return LBM_OK;
 }
@@ -813,9 +848,23 @@ int lbm_load_state(lbm_handle* h, const void* in, size_t bytes)
 {
     int rc = check_handle(h);
     if (rc) return rc;
-    if (!in || bytes != h->pop_bytes())
-        return fail(LBM_ERR_INVALID, "state buffer must be %zu bytes", h->pop_bytes());
-    HIP_TRY(hipMemcpyAsync(h->buf[h->cur], in, bytes, hipMemcpyHostToDevice, h->stream));
+    if (!in || bytes != sizeof(StateHeader) + h->pop_bytes())
+        return fail(LBM_ERR_INVALID, "state buffer must be %zu bytes", sizeof(StateHeader) + h->pop_bytes());
+    StateHeader got;
+    std::memcpy(&got, in, sizeof got);
+    const StateHeader want = state_header(h);
+    if (std::memcmp(got.magic, want.magic, sizeof want.magic) != 0 || got.abi != want.abi)
+        return fail(LBM_ERR_INVALID, "not a state saved by this library (ABI %d)", LBM_ABI_VERSION);
+    if (got.nx != want.nx || got.ny != want.ny || got.mode != want.mode || got.storage != want.storage ||
+        got.layout != want.layout || got.dev_shift != want.dev_shift || got.plane != want.plane ||
+        got.rs != want.rs)
+        return fail(LBM_ERR_INVALID,
+                    "state of a %dx%d lattice (mode %d, storage %d, layout %d, dev_shift %d) does not fit this "
+                    "%dx%d handle (mode %d, storage %d, layout %d, dev_shift %d)",
+                    got.nx, got.ny, got.mode, got.storage, got.layout, got.dev_shift, want.nx, want.ny, want.mode,
+                    want.storage, want.layout, want.dev_shift);
+    HIP_TRY(hipMemcpyAsync(h->buf[h->cur], static_cast<const char*>(in) + sizeof got, h->pop_bytes(),
+                           hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return LBM_OK;
 }

@@ -637,12 +637,19 @@ __device__ inline void project_stress(float d[9])
 // Bounds of the learned modulation alpha of a moment's non-equilibrium part
 // (learned_correction): BGK leaves (1 - omega) n_a after the collision and
 // the correction adds alpha n_a, so |1 - omega + alpha| < 1 needs
-// alpha > omega - 2; the lower bound keeps half of that margin, and the
-// upper bound mirrors it, so clipped noise in the network's output does not
+// omega - 2 < alpha < omega.  The bound b = min((2 - omega)/2, omega/2)
+// keeps half of the nearer margin on both sides: 1 - omega +- b lies in
+// [-omega/2, 2 - 3 omega/2] for omega >= 1 and in [1 - 3 omega/2,
+// 1 - omega/2] below, inside (-1, 1) for every omega in (0, 2).  The
+// interval is symmetric, so clipped noise in the network's output does not
 // shift the effective viscosity on average (near tau = 1/2 a small alpha
 // changes it a lot).
-__host__ __device__ inline float modulation_lo(float omega) { return -0.5f * (2.0f - omega); }
-__host__ __device__ inline float modulation_hi(float omega) { return 0.5f * (2.0f - omega); }
+__host__ __device__ inline float modulation_bound(float omega)
+{
+    return fminf(0.5f * (2.0f - omega), 0.5f * omega);
+}
+__host__ __device__ inline float modulation_lo(float omega) { return -modulation_bound(omega); }
+__host__ __device__ inline float modulation_hi(float omega) { return modulation_bound(omega); }
 
 // The learned collision correction of one cell.  The network's outputs
 // 0, 1, 2 modulate the non-equilibrium parts n_a of the second-order

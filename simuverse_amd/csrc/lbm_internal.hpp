@@ -93,10 +93,11 @@ hipError_t launch_batch_step2(int variant, bool f16, const BatchDomainDev* doms,
 // Learned collision, restriction and health check.
 hipError_t launch_phys_step_learned(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, void* dst,
                                    const uint32_t* bits, uint32_t words, const float* params, float s_in,
-                                   float s_out, hipStream_t s);
+                                   float s_out, float* macro, hipStream_t s);
 hipError_t launch_phys_step_learned_mfma(bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
                                         void* dst, const uint32_t* bits, uint32_t words, const __bf16* packed,
-                                        const float* params, float s_in, float s_out, hipStream_t s);
+                                        const float* params, float s_in, float s_out, float* macro,
+                                        hipStream_t s);
 hipError_t launch_restrict4(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, const uint8_t* mat,
                             float nscale, float gcx, float gcy, float* fc, uint8_t* solid_c, float* uxc, float* uyc,
                             hipStream_t s);

@@ -108,12 +108,15 @@ __global__ __launch_bounds__(256) void k_phys_health(PhysArgs a, const T* __rest
 }
 
 // One step with the learned collision correction (binary32 network, its
-// parameters staged in LDS once per block).  One cell per thread.
+// parameters staged in LDS once per block).  One cell per thread.  macro
+// (or nullptr): the step also exports (ux, uy, rho, w) of the gathered
+// populations, as the plain step does.
 template <typename T, int XEDGE, bool FORCE>
 __global__ __launch_bounds__(256) void k_phys_step_learned(PhysArgs a, const T* __restrict__ src,
                                                            T* __restrict__ dst, const uint32_t* __restrict__ bits,
                                                            uint32_t words, const float* __restrict__ params,
-                                                           float s_in, float s_out, uint32_t nseg)
+                                                           float s_in, float s_out, uint32_t nseg,
+                                                           float* __restrict__ macro)
 {
     __shared__ float P[kMlpParams];
     for (int k = threadIdx.x; k < kMlpParams; k += 256) P[k] = params[k];
@@ -121,8 +124,12 @@ __global__ __launch_bounds__(256) void k_phys_step_learned(PhysArgs a, const T* 
     const int y = (int)(blockIdx.x / nseg);
     const int x = (int)(blockIdx.x % nseg) * 256 + (int)threadIdx.x;
     if (x >= a.nx) return;
-    phys_cell<T, XEDGE, FORCE, false, kNtStore, true>(a, src, dst, bits, words, nullptr,
-                                                      (uint32_t)a.nx * (uint32_t)a.ny, x, y, P, s_in, s_out);
+    const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny;
+    if (macro)
+        phys_cell<T, XEDGE, FORCE, true, kNtStore, true>(a, src, dst, bits, words, macro, n, x, y, P, s_in, s_out);
+    else
+        phys_cell<T, XEDGE, FORCE, false, kNtStore, true>(a, src, dst, bits, words, nullptr, n, x, y, P, s_in,
+                                                          s_out);
 }
 
 // One step with the learned correction evaluated on the matrix cores
@@ -136,7 +143,8 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
                                                                 T* __restrict__ dst, const uint32_t* __restrict__ bits,
                                                                 uint32_t words, const __bf16* __restrict__ packed,
                                                                 const float* __restrict__ params, float s_in,
-                                                                float s_out, uint32_t nseg)
+                                                                float s_out, uint32_t nseg,
+                                                                float* __restrict__ macro)
 {
     __shared__ __align__(16) __bf16 W[kPackElems];
     __shared__ float bias[2 * kMlpH + 16];
@@ -181,6 +189,14 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
             // BGK now, the correction after the network: only g, the three
             // moments and the neighbour bits stay live across it.
             phys_collide<FORCE>(a, g, rho, ux, uy);
+        }
+        if (macro && x < a.nx) {
+            // the macro field of the gathered populations (0 on solids)
+            const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny, c = (uint32_t)y * (uint32_t)a.nx + (uint32_t)x;
+            macro[c] = fluid ? ux : 0.0f;
+            macro[c + n] = fluid ? uy : 0.0f;
+            macro[c + 2 * n] = fluid ? rho : 0.0f;
+            macro[c + 3 * n] = fluid ? 1.0f : 0.0f;
         }
         bf16x8 lo, hi;
 #pragma unroll
@@ -255,13 +271,13 @@ hipError_t launch_phys_health(bool f16, int xedge, bool force, const PhysArgs& a
 
 hipError_t launch_phys_step_learned(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, void* dst,
                                    const uint32_t* bits, uint32_t words, const float* params, float s_in,
-                                   float s_out, hipStream_t s)
+                                   float s_out, float* macro, hipStream_t s)
 {
     const uint32_t nseg = (uint32_t)((a.nx + 255) / 256);
     const dim3 grid(nseg * (uint32_t)a.ny), block(256);
 #define LBM_L(T, XE, FO)                                                                                  \
     hipLaunchKernelGGL((k_phys_step_learned<T, XE, FO>), grid, block, 0, s, a, (const T*)src, (T*)dst, bits, \
-                       words, params, s_in, s_out, nseg)
+                       words, params, s_in, s_out, nseg, macro)
     if (f16) {
         if (xedge) { if (force) LBM_L(_Float16, 1, true); else LBM_L(_Float16, 1, false); }
         else { if (force) LBM_L(_Float16, 0, true); else LBM_L(_Float16, 0, false); }
@@ -275,14 +291,14 @@ hipError_t launch_phys_step_learned(bool f16, int xedge, bool force, const PhysA
 
 hipError_t launch_phys_step_learned_mfma(bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
                                         void* dst, const uint32_t* bits, uint32_t words, const __bf16* packed,
-                                        const float* params, float s_in, float s_out, hipStream_t s)
+                                        const float* params, float s_in, float s_out, float* macro, hipStream_t s)
 {
     const uint32_t nseg = (uint32_t)((a.nx + 255) / 256);
     const uint32_t tiles = nseg * (uint32_t)a.ny;
     const dim3 grid(tiles < 2048u ? tiles : 2048u), block(256);
 #define LBM_L(T, XE, FO)                                                                                       \
     hipLaunchKernelGGL((k_phys_step_learned_mfma<T, XE, FO>), grid, block, 0, s, a, (const T*)src, (T*)dst, bits, \
-                       words, packed, params, s_in, s_out, nseg)
+                       words, packed, params, s_in, s_out, nseg, macro)
     if (f16) {
         if (xedge) { if (force) LBM_L(_Float16, 1, true); else LBM_L(_Float16, 1, false); }
         else { if (force) LBM_L(_Float16, 0, true); else LBM_L(_Float16, 0, false); }

@@ -411,6 +411,28 @@ def test_checkpoint_resume_bit_identical():
     d.run(73)
     fl = ~solid
     np.testing.assert_allclose(d.read_populations()[:, fl], ref[:, fl], rtol=0, atol=1e-6)
+    # A state carries a header: one saved from a handle whose populations
+    # mean something else (another layout, storage scale or size) is
+    # rejected, not loaded as wrong values.
+    planes = L.Lattice(nx, ny, 0.6, x_edge=L.X_INLET_OUTLET, inlet_ux=0.05, layout=L.LAYOUT_PLANES)
+    with pytest.raises(L.LbmError) as ei:
+        planes.load_state(ck)
+    assert ei.value.code == 1
+    assert "does not fit" in str(ei.value) or "bytes" in str(ei.value)
+    h14 = L.Lattice(nx, ny, 0.6, storage=L.STORAGE_F16_DEV, dev_shift=14)
+    st14 = h14.save_state()
+    h10 = L.Lattice(nx, ny, 0.6, storage=L.STORAGE_F16_DEV)
+    assert st14.size == h10.save_state().size
+    with pytest.raises(L.LbmError) as ei:
+        h10.load_state(st14)
+    assert "dev_shift 14" in str(ei.value)
+    bad = ck.copy()
+    bad[:8] = 0
+    with pytest.raises(L.LbmError) as ei:
+        b.load_state(bad)
+    assert "not a state" in str(ei.value)
+    for h in (planes, h14, h10):
+        h.close()
 
 
 def test_f16_deviation_storage_vs_fp64():

@@ -390,3 +390,160 @@ def test_learned_coarse_beats_plain_coarse():
           flush=True)
     assert np.isfinite(errs["learned"])
     assert errs["learned"] < 0.8 * errs["plain"]
+
+
+# ---------------------------------------------------------------------------
+# The learned step, its restriction and its training targets against the
+# binary64 oracle (oracle/learned_oracle.c)
+# ---------------------------------------------------------------------------
+
+def walled_scene(kind, nx, ny):
+    """A channel (Zou-He inlet 0.04, zero-gradient outlet, walls at y = 0
+    and ny - 1, a cylinder, body force) or a periodic lattice with disks and
+    a body force."""
+    if kind == "channel":
+        solid = W.cylinder_channel_solid(nx, ny, cx=nx // 4, cy=ny // 2, r=ny // 8)
+        return solid, dict(x_edge=L.X_INLET_OUTLET, inlet_ux=0.04, force=(1e-5, 0.0))
+    return W.periodic_disks(nx, ny, 0.06, 3, 7, seed=17).solid, dict(force=(2e-5, -1e-5))
+
+
+def device_learned_run(solid, f0, tau, kw, net, precision, steps, storage=L.STORAGE_F32):
+    ny, nx = solid.shape
+    lat = L.Lattice(nx, ny, tau, storage=storage, **kw)
+    lat.write_materials(W.materials_from_solid(solid))
+    lat.write_populations(f0.astype(np.float32), 0)
+    if net is not None:
+        lat.set_learned_collision(net)
+        lat.set_learned_precision(precision)
+    lat.run(steps)
+    f = lat.read_populations().astype(np.float64)
+    lat.close()
+    return f
+
+
+@pytest.mark.parametrize("kind", ["channel", "periodic"])
+def test_learned_step_matches_fp64_oracle(kind):
+    """k_phys_step_learned (binary32 network) and k_phys_step_learned_mfma
+    (bf16 operands, binary32 accumulation) against the binary64 oracle of
+    the learned step, on lattices with walls, after 1 and 20 steps from a
+    perturbed state.  Bounds: binary32, |f - f_oracle| <= 2e-7 on every
+    fluid population (f ~ 0.1; the correction itself changes f by ~1e-4);
+    bf16, the difference from the oracle <= 5 % of what the correction
+    changes (rms over fluid populations), the rounding of the network's
+    operands to 8 significant bits."""
+    nx, ny = 96, 64
+    tau = 0.56
+    solid, kw = walled_scene(kind, nx, ny)
+    f0 = W.rest_state(nx, ny, solid, amplitude=1e-3, seed=12)
+    p64 = random_params(10, 0.2)
+    net = L.Network(seed=6)
+    net.set_params(p64)
+    s_in, s_out = 3e-2, 0.05
+    net.set_scales(s_in, s_out)
+    po = O.phys_params(nx, ny, tau, x_edge=kw.get("x_edge", 0), inlet_ux=kw.get("inlet_ux", 0.0),
+                       force=kw["force"])
+    fl = ~solid
+    for steps in (1, 20):
+        print(f"case: learned step vs fp64 oracle, {kind}, {steps} steps", flush=True)
+        ref = O.learned_run(po, solid, f0, p64.astype(np.float64), s_in, s_out, steps)
+        plain = O.PhysState(po, solid, f0)
+        plain.run(steps)
+        effect = ref[:, fl] - plain.f[:, fl]
+        assert np.sqrt(np.mean(effect**2)) > 1e-6
+        f32 = device_learned_run(solid, f0, tau, kw, net, L.MLP_F32, steps)
+        err32 = np.max(np.abs(f32[:, fl] - ref[:, fl]))
+        bf = device_learned_run(solid, f0, tau, kw, net, L.MLP_BF16, steps)
+        errbf = np.sqrt(np.mean((bf[:, fl] - ref[:, fl]) ** 2)) / np.sqrt(np.mean(effect**2))
+        print(f"f32 max err {err32:.3e}; bf16 rms err / correction {errbf:.3e}; "
+              f"correction rms {np.sqrt(np.mean(effect**2)):.3e}", flush=True)
+        assert err32 <= 2e-7
+        assert errbf <= 5e-2
+
+
+@pytest.mark.parametrize("tau", [0.505, 2.0])
+def test_learned_step_saturated_modulation_matches_oracle(tau):
+    """A network whose outputs saturate the clamp (b = min((2 - omega)/2,
+    omega/2)): at tau = 2 the upper bound is omega/2, which keeps the
+    relaxation factors inside (-1, 1); device and oracle agree there and at
+    tau = 0.505, and the state stays bounded over 200 steps."""
+    nx, ny = 64, 48
+    solid, kw = walled_scene("periodic", nx, ny)
+    f0 = W.rest_state(nx, ny, solid, amplitude=1e-3, seed=2)
+    p = np.zeros(5385, np.float32)
+    p[5376:5379] = (40.0, -40.0, 40.0)  # b3: saturates alpha at +b, -b, +b
+    net = L.Network(seed=1)
+    net.set_params(p)
+    net.set_scales(1.0, 1.0)
+    po = O.phys_params(nx, ny, tau, force=kw["force"])
+    fl = ~solid
+    for prec in (L.MLP_F32, L.MLP_BF16):
+        print(f"case: saturated modulation, tau {tau}, precision {prec}", flush=True)
+        ref = O.learned_run(po, solid, f0, p.astype(np.float64), 1.0, 1.0, 10)
+        dev = device_learned_run(solid, f0, tau, kw, net, prec, 10)
+        assert np.max(np.abs(dev[:, fl] - ref[:, fl])) <= 2e-7
+    lat = L.Lattice(nx, ny, tau, **kw)
+    lat.write_materials(W.materials_from_solid(solid))
+    lat.write_populations(f0.astype(np.float32), 0)
+    lat.set_learned_collision(net)
+    lat.run(200)
+    u, finite = lat.health()
+    assert finite and u < 0.1
+
+
+def test_restriction_matches_fp64_oracle():
+    """lbm_restrict4 against the oracle's restriction of the same fine state
+    (periodic disks, body force, 300 steps in): coarse populations within
+    2e-6 of f (binary32 sums of 16 cells), the solid map exactly, the coarse
+    velocity within 1e-9 absolute."""
+    print("case: restriction vs fp64 oracle", flush=True)
+    nx, ny = 128, 96
+    scene = W.periodic_disks(nx, ny, 0.08, 4, 10, seed=2)
+    g, gc = (2e-5, 0.0), (8e-5, 0.0)
+    lat = L.Lattice(nx, ny, 0.52, force=g)
+    lat.write_materials(W.materials_from_solid(scene.solid))
+    lat.run(300)
+    post = lat.read_populations().astype(np.float64)
+    fc, sc, uxc, uyc = lat.restrict4(0.505, gc)
+    lat.close()
+    po = O.phys_params(nx, ny, 0.52, force=g)
+    rfc, rsc, rux, ruy = O.restrict4_state(po, scene.solid, post, 0.505, gc)
+    assert np.array_equal(sc, rsc)
+    assert np.max(np.abs(fc - rfc)) <= 2e-6
+    assert np.max(np.abs(uxc - rux)) <= 1e-9 + 1e-5 * np.max(np.abs(rux))
+    assert np.max(np.abs(uyc - ruy)) <= 1e-9 + 1e-5 * np.max(np.abs(rux))
+    # the fine state carries a non-equilibrium part the restriction rescales
+    assert np.max(np.abs(rux)) > 1e-4
+
+
+@pytest.mark.parametrize("storage", [L.STORAGE_F32, L.STORAGE_F16_DEV], ids=["f32", "f16"])
+@pytest.mark.parametrize("precision", [L.MLP_BF16, L.MLP_F32], ids=["bf16", "f32"])
+def test_learned_handle_exports_macro(storage, precision):
+    """FluidSimulator::compute on a learned lattice: lbm_run(h, 2, 1) exports
+    the macro field of the last step's gathered populations, which equals
+    lbm_macro of the state before that step bit for bit (0 on solids, w = 1
+    on fluid); the rgba16float readback, through the handle's staging
+    buffer, is the same on every call."""
+    print(f"case: learned handle macro export, storage {storage}, precision {precision}", flush=True)
+    nx, ny = 96, 64
+    solid, kw = walled_scene("channel", nx, ny)
+    f0 = W.rest_state(nx, ny, solid, amplitude=1e-3, seed=12).astype(np.float32)
+    net = L.Network(seed=6)
+    net.set_params(random_params(10, 0.2))
+    net.set_scales(3e-2, 0.05)
+    lat = L.Lattice(nx, ny, 0.56, storage=storage, **kw)
+    lat.write_materials(W.materials_from_solid(solid))
+    lat.write_populations(f0, 0)
+    lat.set_learned_collision(net)
+    lat.set_learned_precision(precision)
+    for frame in range(3):
+        lat.run(1)
+        rho0, ux0, uy0 = lat.macro()
+        lat.run(1, export_macro=True)
+        rho, ux, uy = lat.read_macro()
+        assert np.array_equal(rho, rho0) and np.array_equal(ux, ux0) and np.array_equal(uy, uy0), frame
+        assert np.all(rho[solid] == 0) and np.all(rho[~solid] > 0.9)
+    t1 = lat.read_macro_rgba16f()
+    t2 = lat.read_macro_rgba16f()
+    assert np.array_equal(t1, t2)
+    assert np.array_equal(t1[..., 3] == 0, solid)
+    lat.close()
