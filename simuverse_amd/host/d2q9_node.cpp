@@ -71,6 +71,75 @@ std::vector<LatticeInfo> init_lattice_material(uint32_t nx, uint32_t ny, FieldAn
 }
 
 // ---------------------------------------------------------------------------
+// Edit geometry
+// ---------------------------------------------------------------------------
+
+CellRange obstacle_band(std::vector<LatticeInfo>& info, uint32_t nx, uint32_t ny, uint32_t x, uint32_t y)
+{
+    const uint32_t r = (uint32_t)OBSTACLE_RADIUS;
+    if (y < r || y - r + 2 * r > ny || info.size() != (size_t)nx * ny)
+        throw LbmError(LBM_ERR_INVALID, "add_obstacle: the disk's row band leaves the lattice");
+    const LatticeInfo obstacle{(int32_t)LatticeType::Obstacle, -1, 0.0f, 0.0f};
+    const Vec2 center{(float)x + 0.5f, (float)y + 0.5f};
+    const uint32_t min_y = y - r, max_y = min_y + 2 * r;
+    for (uint32_t yy = min_y; yy < max_y; ++yy)
+        for (uint32_t xx = 0; xx < nx; ++xx)
+            if (is_sd_sphere(Vec2{(float)xx + 0.5f - center.x, (float)yy + 0.5f - center.y}, OBSTACLE_RADIUS))
+                info[(size_t)yy * nx + xx] = obstacle;
+    // The whole band goes up: cells outside the disk carry the host copy.
+    return CellRange{nx * min_y, nx * (max_y - min_y)};
+}
+
+ForceStroke force_stroke(uint32_t nx, uint32_t ny, uint32_t pixel, Vec2 pos, Vec2 pre_pos)
+{
+    if (pixel < 2) throw LbmError(LBM_ERR_INVALID, "add_external_force: lattice_pixel_size must be >= 2");
+    const float dx = pos.x - pre_pos.x, dy = pos.y - pre_pos.y;
+    const float dis = std::sqrt(dx * dx + dy * dy);
+    float force = 0.1f * (dis / 20.0f);
+    if (force > 0.12f) force = 0.12f;
+    const float ridian = std::atan2(dy, dx);
+    ForceStroke out;
+    out.record = LatticeInfo{(int32_t)LatticeType::ExternalForce, 90, force * std::cos(ridian),
+                             force * std::sin(ridian)};
+    const float c = std::ceil(dis / (float)(pixel - 1));
+    const float step = dis / c;
+    for (int i = 0; i < (int)c; ++i) {
+        const float px = std::round(pre_pos.x + step * (float)i * std::cos(ridian));
+        const float py = std::round(pre_pos.y + step * (float)i * std::sin(ridian));
+        // `as u32` saturates: negatives and NaN to 0
+        const uint32_t x = (px > 0.0f ? (px < 4294967296.0f ? (uint32_t)px : 0xFFFFFFFFu) : 0u) / pixel;
+        const uint32_t y = (py > 0.0f ? (py < 4294967296.0f ? (uint32_t)py : 0xFFFFFFFFu) : 0u) / pixel;
+        if (x < 1 || x >= nx - 2 || y < 1 || y >= ny - 2) continue;
+        out.cells.push_back(nx * y + x);
+    }
+    return out;
+}
+
+bool click_cell(uint32_t nx, uint32_t ny, uint32_t pixel, Vec2 pos, uint32_t* x, uint32_t* y)
+{
+    if (pos.x <= 0.0f || pos.y <= 0.0f) return false;
+    const uint32_t cx = (uint32_t)pos.x / pixel;
+    const uint32_t cy = (uint32_t)pos.y / pixel;
+    const uint32_t half = (uint32_t)OBSTACLE_RADIUS;
+    if (cx < half || cx >= nx - (half + 2) || cy < half || cy >= ny - (half + 2)) return false;
+    *x = cx;
+    *y = cy;
+    return true;
+}
+
+bool touch_move_draws(Vec2 pre_pos, Vec2 pos, Vec2* next)
+{
+    if (pos.x <= 0.0f || pos.y <= 0.0f) {
+        *next = Vec2{};
+        return false;
+    }
+    const float dx = pos.x - pre_pos.x, dy = pos.y - pre_pos.y;
+    const float dis = std::sqrt(dx * dx + dy * dy);
+    *next = pos;
+    return !((pre_pos.x == 0.0f && pre_pos.y == 0.0f) || dis > 300.0f);
+}
+
+// ---------------------------------------------------------------------------
 // D2Q9Node
 // ---------------------------------------------------------------------------
 
@@ -124,41 +193,14 @@ void D2Q9Node::reset_lattice_info()
 
 void D2Q9Node::add_obstacle(uint32_t x, uint32_t y)
 {
-    const uint32_t r = (uint32_t)OBSTACLE_RADIUS;
-    if (y < r || y - r + 2 * r > ny_)
-        throw LbmError(LBM_ERR_INVALID, "add_obstacle: the disk's row band leaves the lattice");
-    const LatticeInfo obstacle{(int32_t)LatticeType::Obstacle, -1, 0.0f, 0.0f};
-    const Vec2 center{(float)x + 0.5f, (float)y + 0.5f};
-    const uint32_t min_y = y - r, max_y = min_y + 2 * r;
-    for (uint32_t yy = min_y; yy < max_y; ++yy)
-        for (uint32_t xx = 0; xx < nx_; ++xx)
-            if (is_sd_sphere(Vec2{(float)xx + 0.5f - center.x, (float)yy + 0.5f - center.y}, OBSTACLE_RADIUS))
-                info_[(size_t)yy * nx_ + xx] = obstacle;
-    // The whole band goes up: cells outside the disk get the host copy back
-    // (d2q9_node.rs:236-244).
-    write_info_rows(nx_ * min_y, nx_ * (max_y - min_y));
+    const CellRange band = obstacle_band(info_, nx_, ny_, x, y);
+    write_info_rows(band.first, band.count);
 }
 
 void D2Q9Node::add_external_force(Vec2 pos, Vec2 pre_pos)
 {
-    if (pixel_ < 2) throw LbmError(LBM_ERR_INVALID, "add_external_force: lattice_pixel_size must be >= 2");
-    const float dx = pos.x - pre_pos.x, dy = pos.y - pre_pos.y;
-    const float dis = std::sqrt(dx * dx + dy * dy);
-    float force = 0.1f * (dis / 20.0f);
-    if (force > 0.12f) force = 0.12f;
-    const float ridian = std::atan2(dy, dx);
-    const LatticeInfo rec{(int32_t)LatticeType::ExternalForce, 90, force * std::cos(ridian), force * std::sin(ridian)};
-    const float c = std::ceil(dis / (float)(pixel_ - 1));
-    const float step = dis / c;
-    for (int i = 0; i < (int)c; ++i) {
-        const float px = std::round(pre_pos.x + step * (float)i * std::cos(ridian));
-        const float py = std::round(pre_pos.y + step * (float)i * std::sin(ridian));
-        // `as u32` saturates negatives to 0
-        const uint32_t x = (px > 0.0f ? (uint32_t)px : 0u) / pixel_;
-        const uint32_t y = (py > 0.0f ? (uint32_t)py : 0u) / pixel_;
-        if (x < 1 || x >= nx_ - 2 || y < 1 || y >= ny_ - 2) continue;
-        check(lbm_write_lattice_info(h_, &rec, (size_t)nx_ * y + x, 1));
-    }
+    const ForceStroke stroke = force_stroke(nx_, ny_, pixel_, pos, pre_pos);
+    for (uint32_t cell : stroke.cells) check(lbm_write_lattice_info(h_, &stroke.record, cell, 1));
 }
 
 void D2Q9Node::compute_by_pass(int swap_index) { check(lbm_step(h_, swap_index, 1)); }
@@ -183,30 +225,17 @@ FluidSimulator::FluidSimulator(uint32_t canvas_width, uint32_t canvas_height, ui
 
 void FluidSimulator::on_click(Vec2 pos)
 {
-    if (pos.x <= 0.0f || pos.y <= 0.0f) return;
-    const uint32_t x = (uint32_t)pos.x / node_.lattice_pixel_size();
-    const uint32_t y = (uint32_t)pos.y / node_.lattice_pixel_size();
-    const uint32_t half = (uint32_t)OBSTACLE_RADIUS;
-    if (x < half || x >= node_.width() - (half + 2) || y < half || y >= node_.height() - (half + 2)) return;
-    node_.add_obstacle(x, y);
+    uint32_t x = 0, y = 0;
+    if (click_cell(node_.width(), node_.height(), node_.lattice_pixel_size(), pos, &x, &y)) node_.add_obstacle(x, y);
 }
 
 void FluidSimulator::touch_begin() { pre_pos_ = Vec2{}; }
 
 void FluidSimulator::touch_move(Vec2 pos)
 {
-    if (pos.x <= 0.0f || pos.y <= 0.0f) {
-        pre_pos_ = Vec2{};
-        return;
-    }
-    const float dx = pos.x - pre_pos_.x, dy = pos.y - pre_pos_.y;
-    const float dis = std::sqrt(dx * dx + dy * dy);
-    if ((pre_pos_.x == 0.0f && pre_pos_.y == 0.0f) || dis > 300.0f) {
-        pre_pos_ = pos;
-        return;
-    }
-    node_.add_external_force(pos, pre_pos_);
-    pre_pos_ = pos;
+    Vec2 next;
+    if (touch_move_draws(pre_pos_, pos, &next)) node_.add_external_force(pos, pre_pos_);
+    pre_pos_ = next;
 }
 
 void FluidSimulator::update_uniforms(const SettingObj& setting) { node_.update_uniforms(setting); }

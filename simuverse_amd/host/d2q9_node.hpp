@@ -71,6 +71,41 @@ bool is_sd_sphere(Vec2 p, float r);
 // lattice.rs:26-98: materials of an nx x ny lattice for an animation type.
 std::vector<LatticeInfo> init_lattice_material(uint32_t nx, uint32_t ny, FieldAnimationType ty);
 
+// ---- edit geometry (host only, no device calls) ---------------------------
+
+// Cells [first, first + count) of a row-major lattice.
+struct CellRange {
+    uint32_t first = 0, count = 0;
+};
+
+// d2q9_node.rs:215-245: rows y - r .. y + r - 1 (r = OBSTACLE_RADIUS) of the
+// host copy `info` (nx*ny records) get an Obstacle record where
+// |(xx + 0.5, yy + 0.5) - (x + 0.5, y + 0.5)| <= r; returns the band the
+// reference uploads (whole rows, cells outside the disk with their host
+// value).  The band must lie inside the lattice (y >= r, y + r <= ny).
+CellRange obstacle_band(std::vector<LatticeInfo>& info, uint32_t nx, uint32_t ny, uint32_t x, uint32_t y);
+
+// d2q9_node.rs:263-300: the record a drag from pre_pos to pos (canvas
+// pixels) writes (ExternalForce, block_iter 90, force 0.1 |d| / 20 capped at
+// 0.12 along the drag) and the cells it writes it to, in order: ceil(|d| /
+// (pixel - 1)) points spaced |d| / that count from pre_pos, rounded half away
+// from zero, converted as Rust's saturating `as u32`, divided by pixel,
+// skipped outside 1 <= x < nx - 2, 1 <= y < ny - 2.  pixel >= 2.
+struct ForceStroke {
+    LatticeInfo record{};
+    std::vector<uint32_t> cells;
+};
+ForceStroke force_stroke(uint32_t nx, uint32_t ny, uint32_t pixel, Vec2 pos, Vec2 pre_pos);
+
+// fluid_simulator.rs:137-152: the lattice cell a click at canvas pos adds an
+// obstacle at, or false when the click is outside the allowed band.
+bool click_cell(uint32_t nx, uint32_t ny, uint32_t pixel, Vec2 pos, uint32_t* x, uint32_t* y);
+
+// fluid_simulator.rs:158-173: given the previous drag position, whether a
+// move to pos draws a stroke (and so calls add_external_force); *next is
+// the new previous position.
+bool touch_move_draws(Vec2 pre_pos, Vec2 pos, Vec2* next);
+
 // The settings D2Q9Node::new reads (d2q9_node.rs:31-59).
 struct SettingObj {
     float fluid_viscosity = 0.02f;
