@@ -359,6 +359,9 @@ int lbm_mlp_loss_grad_noalloc(lbm_mlp* m, const float* x, const float* t, int n,
 /* Training samples held on the device. */
 int lbm_mlp_clear_samples(lbm_mlp* m);
 int lbm_mlp_sample_count(lbm_mlp* m, int* n);
+/* Copies stored samples offset .. offset+count-1 to the host (x, t: count*9
+ * floats each; either may be NULL), in the order they were stored. */
+int lbm_mlp_read_samples(lbm_mlp* m, int offset, int count, float* x, float* t);
 int lbm_mlp_add_samples(lbm_mlp* m, const float* x, const float* t, int n);
 /* Samples from two consecutive pre-collision states of a periodic lattice
  * (dense f[i*nx*ny + c], binary32): for every stride-th cell from offset
@@ -383,7 +386,11 @@ int lbm_mlp_train(lbm_mlp* m, int steps, int batch, float lr, uint64_t seed, dou
 
 /* Physics-mode handles: steps add the network's projected correction to
  * the BGK collision; NULL switches back to plain BGK.  The network must
- * outlive its use. */
+ * outlive its use.  Changing the network's parameters or scales later
+ * (lbm_mlp_set_params, lbm_mlp_set_scales, lbm_mlp_normalize_samples,
+ * lbm_mlp_train) is seen by every later step of every attached handle: the
+ * parameter updates first wait for all work queued on the device, and
+ * handles re-capture their HIP-graph chunks. */
 int lbm_set_learned_collision(lbm_handle* h, const lbm_mlp* m);
 /* Arithmetic of the learned step's network: LBM_MLP_BF16 (matrix cores,
  * bf16 operands, binary32 accumulation; default) or LBM_MLP_F32 (binary32

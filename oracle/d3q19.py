@@ -76,7 +76,10 @@ class D3Q19State:
     def moments(self, pre: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
         rho = pre.sum(0)
         j = np.tensordot(E.T.astype(np.float64), pre, axes=(1, 0))
-        u = (j + 0.5 * self.g[:, None, None, None]) / rho
+        # solid cells hold rho = 0: their velocity is defined as 0, not j/0
+        fluid = rho != 0.0
+        u = np.zeros_like(j)
+        np.divide(j + 0.5 * self.g[:, None, None, None], rho, out=u, where=fluid)
         return rho, u
 
     def step(self) -> None:

@@ -89,6 +89,7 @@ struct lbm_handle {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     const lbm_mlp* mlp = nullptr;   // learned collision correction, if set
+    uint64_t graph_mlp_gen = 0;     // the network's generation the graphs were captured with
     int mlp_precision = LBM_MLP_BF16;
     uint32_t* health = nullptr;     // [max |u|^2 bits, non-finite flag]
     void* backup = nullptr;         // chunk start state of lbm_run_guarded
@@ -184,6 +185,11 @@ int chunk_flips(const lbm_handle* h) { return two_step(h) ? (h->graph_chunk / 2)
 
 int ensure_graph(lbm_handle* h, int b)
 {
+    // graphs of learned steps hold the network's scales as launch arguments
+    if (h->mlp && lbm_mlp_generation(h->mlp) != h->graph_mlp_gen) {
+        drop_graphs(h);
+        h->graph_mlp_gen = lbm_mlp_generation(h->mlp);
+    }
     if (h->graph[b]) return LBM_OK;
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     hipError_t launch_err = hipSuccess;

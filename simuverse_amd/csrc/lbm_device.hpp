@@ -517,6 +517,31 @@ __device__ __forceinline__ bf16x8 gelu8_bf16(const float* v)
 
 __device__ __forceinline__ bf16x8 lds_b128(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
 
+// LDS bias block of the matrix-core network: b1[64], b2[64], then b3 padded
+// with zeros to the 32 rows of its M tile (16-byte aligned).
+constexpr int kBiasElems = 2 * kMlpH + 32;
+
+// The biases of one 32-row M tile as its initial accumulator: lane half h
+// holds rows 8g + 4h .. 8g + 4h + 3 (c_row), four 16-byte LDS reads.  The
+// row offset passes through an empty asm statement, so every tile reads its
+// biases afresh straight into the accumulator registers instead of reusing
+// one read and copying it (which cost a vector move per value).
+__device__ __forceinline__ f32x16 bias_tile(const float* __restrict__ b, int h)
+{
+    int off = 4 * h;
+    asm volatile("" : "+v"(off));
+    f32x16 c;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const float4 v = *reinterpret_cast<const float4*>(b + off + 8 * g);
+        c[4 * g] = v.x;
+        c[4 * g + 1] = v.y;
+        c[4 * g + 2] = v.z;
+        c[4 * g + 3] = v.w;
+    }
+    return c;
+}
+
 // Network on the 64 samples of one wavefront.  W: packed weights (LDS);
 // bias: b1[64], b2[64], b3[9] in binary32 (LDS); xs: 64 x 16 bf16 inputs
 // (row per sample, inputs 9..15 zero); ys: 64 x 9 binary32 outputs.  Every
@@ -534,9 +559,7 @@ __device__ __forceinline__ void mlp_wave_bf16(const __bf16* __restrict__ W, cons
         bf16x8 hb[4];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
-            f32x16 c;
-#pragma unroll
-            for (int j = 0; j < 16; ++j) c[j] = b1[32 * mt + c_row(j, h)];
+            f32x16 c = bias_tile(b1 + 32 * mt, h);
             c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b128(W + kPackW1 + (mt * 64 + lane) * 8), xb, c, 0, 0, 0);
 {
                 float v[16];
@@ -549,9 +572,7 @@ __device__ __forceinline__ void mlp_wave_bf16(const __bf16* __restrict__ W, cons
         bf16x8 hb2[4];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
-            f32x16 c;
-#pragma unroll
-            for (int j = 0; j < 16; ++j) c[j] = b2[32 * mt + c_row(j, h)];
+            f32x16 c = bias_tile(b2 + 32 * mt, h);
 #pragma unroll
             for (int s = 0; s < 4; ++s)
                 c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b128(W + kPackW2 + ((mt * 4 + s) * 64 + lane) * 8),
@@ -564,12 +585,8 @@ __device__ __forceinline__ void mlp_wave_bf16(const __bf16* __restrict__ W, cons
                 hb2[2 * mt + 1] = gelu8_bf16(v + 8);
             }
         }
-        f32x16 c;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int o = c_row(j, h);
-            c[j] = o < kMlpOut ? b3[o] : 0.0f;
-        }
+        // b3 is stored padded with zeros to 32 rows
+        f32x16 c = bias_tile(b3, h);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
             c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b128(W + kPackW3 + (s * 64 + lane) * 8), hb2[s], c, 0, 0,

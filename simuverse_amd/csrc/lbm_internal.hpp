@@ -112,6 +112,8 @@ const float* lbm_mlp_device_params(const lbm_mlp* m);
 // Device pointer to its lane-ordered bf16 weights (kPackElems).
 const __bf16* lbm_mlp_device_packed(const lbm_mlp* m);
 void lbm_mlp_scales(const lbm_mlp* m, float* s_in, float* s_out);
+// Changes whenever the network's parameters or scales change.
+uint64_t lbm_mlp_generation(const lbm_mlp* m);
 
 namespace lbm {
 

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
                                                                 float* __restrict__ macro)
 {
     __shared__ __align__(16) __bf16 W[kPackElems];
-    __shared__ float bias[2 * kMlpH + 16];
+    __shared__ __align__(16) float bias[kBiasElems];
     __shared__ __align__(16) __bf16 xs[4][64 * 16];
     __shared__ float ys[4][64 * kMlpOut];
     {
@@ -158,7 +158,8 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
             bias[k] = params[kOffB1 + k];
             bias[kMlpH + k] = params[kOffB2 + k];
         }
-        if (threadIdx.x < kMlpOut) bias[2 * kMlpH + threadIdx.x] = params[kOffB3 + threadIdx.x];
+        if (threadIdx.x < kBiasElems - 2 * kMlpH)
+            bias[2 * kMlpH + threadIdx.x] = threadIdx.x < kMlpOut ? params[kOffB3 + threadIdx.x] : 0.0f;
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;

@@ -77,7 +77,8 @@ __device__ inline void stage_bf16_net(const __bf16* __restrict__ packed, const f
         bias[k] = params[kOffB1 + k];
         bias[kMlpH + k] = params[kOffB2 + k];
     }
-    for (int k = threadIdx.x; k < kMlpOut; k += blockDim.x) bias[2 * kMlpH + k] = params[kOffB3 + k];
+    for (int k = threadIdx.x; k < kBiasElems - 2 * kMlpH; k += blockDim.x)
+        bias[2 * kMlpH + k] = k < kMlpOut ? params[kOffB3 + k] : 0.0f;
 }
 
 __global__ __launch_bounds__(256) void k_mlp_forward_bf16(const __bf16* __restrict__ packed,
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward_bf16(const __bf16* __restri
                                                           const float* __restrict__ X, float* __restrict__ Y, int n)
 {
     __shared__ __align__(16) __bf16 W[kPackElems];
-    __shared__ float bias[2 * kMlpH + 16];
+    __shared__ __align__(16) float bias[kBiasElems];
     __shared__ __align__(16) __bf16 xs[4][64 * 16];
     __shared__ float ys[4][64 * kMlpOut];
     stage_bf16_net(packed, params, W, bias);
@@ -429,6 +430,10 @@ struct lbm_mlp {
     int samples = 0;
     int adam_step = 0;
     float s_in = 1.0f, s_out = 1.0f;
+    // Bumped whenever the parameters or scales change: lattices with this
+    // network attached rebuild the HIP graphs they captured (a graph holds
+    // the scales its launches were recorded with).
+    uint64_t generation = 0;
 };
 
 namespace {
@@ -437,6 +442,15 @@ int mlp_check(lbm_mlp* m)
 {
     if (!m) return lbm_fail(LBM_ERR_INVALID, "null network");
     MLP_TRY(hipSetDevice(m->dev));
+    return LBM_OK;
+}
+
+// Before the parameters change: lattices on other streams may still be
+// running learned steps that read them, so every queue of the device drains
+// first.  (Updates are rare next to steps; the wait costs one round trip.)
+int mlp_quiesce_device(lbm_mlp* m)
+{
+    MLP_TRY(hipDeviceSynchronize());
     return LBM_OK;
 }
 
@@ -493,6 +507,7 @@ void lbm_mlp_scales(const lbm_mlp* m, float* s_in, float* s_out)
     *s_in = m->s_in;
     *s_out = m->s_out;
 }
+uint64_t lbm_mlp_generation(const lbm_mlp* m) { return m->generation; }
 
 extern "C" {
 
@@ -563,6 +578,8 @@ int lbm_mlp_set_params(lbm_mlp* m, const float* params)
     int rc = mlp_check(m);
     if (rc) return rc;
     if (!params) return lbm_fail(LBM_ERR_INVALID, "null params");
+    if ((rc = mlp_quiesce_device(m))) return rc;
+    ++m->generation;
     MLP_TRY(hipMemcpyAsync(m->params, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, m->stream));
     MLP_TRY(hipMemsetAsync(m->adam_m, 0, kMlpParams * sizeof(float), m->stream));
     MLP_TRY(hipMemsetAsync(m->adam_v, 0, kMlpParams * sizeof(float), m->stream));
@@ -587,6 +604,7 @@ int lbm_mlp_set_scales(lbm_mlp* m, float in_scale, float out_scale)
     if (!m) return lbm_fail(LBM_ERR_INVALID, "null network");
     if (!(in_scale > 0.0f) || !(out_scale > 0.0f) || !std::isfinite(in_scale) || !std::isfinite(out_scale))
         return lbm_fail(LBM_ERR_INVALID, "scales must be finite and > 0");
+    if (in_scale != m->s_in || out_scale != m->s_out) ++m->generation;
     m->s_in = in_scale;
     m->s_out = out_scale;
     return LBM_OK;
@@ -779,6 +797,7 @@ int lbm_mlp_normalize_samples(lbm_mlp* m, float* in_rms, float* out_rms)
     MLP_TRY(hipStreamSynchronize(m->stream));
     m->s_in *= rx;
     m->s_out *= rt;
+    ++m->generation;
     if (in_rms) *in_rms = rx;
     if (out_rms) *out_rms = rt;
     return LBM_OK;
@@ -788,6 +807,23 @@ int lbm_mlp_sample_count(lbm_mlp* m, int* n)
 {
     if (!m || !n) return lbm_fail(LBM_ERR_INVALID, "null argument");
     *n = m->samples;
+    return LBM_OK;
+}
+
+int lbm_mlp_read_samples(lbm_mlp* m, int offset, int count, float* x, float* t)
+{
+    int rc = mlp_check(m);
+    if (rc) return rc;
+    if (offset < 0 || count < 0 || offset > m->samples || count > m->samples - offset)
+        return lbm_fail(LBM_ERR_INVALID, "samples [%d, %d) outside the %d stored", offset, offset + count, m->samples);
+    if (count == 0) return LBM_OK;
+    if (x)
+        MLP_TRY(hipMemcpyAsync(x, m->X + (size_t)offset * kMlpIn, (size_t)count * kMlpIn * sizeof(float),
+                               hipMemcpyDeviceToHost, m->stream));
+    if (t)
+        MLP_TRY(hipMemcpyAsync(t, m->T + (size_t)offset * kMlpOut, (size_t)count * kMlpOut * sizeof(float),
+                               hipMemcpyDeviceToHost, m->stream));
+    MLP_TRY(hipStreamSynchronize(m->stream));
     return LBM_OK;
 }
 
@@ -876,6 +912,8 @@ int lbm_mlp_train(lbm_mlp* m, int steps, int batch, float lr, uint64_t seed, dou
     if (rc) return rc;
     if (steps < 0 || batch < 1 || !(lr > 0.0f)) return lbm_fail(LBM_ERR_INVALID, "bad arguments");
     if (m->samples < 1) return lbm_fail(LBM_ERR_INVALID, "no samples to train on");
+    if ((rc = mlp_quiesce_device(m))) return rc;
+    ++m->generation;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     for (int s = 0; s < steps; ++s) {
         rc = mlp_grad(m, m->X, m->T, m->samples, batch, seed + 0xD1B54A32D192ED03ull * (uint64_t)(m->adam_step + 1), 1);

@@ -128,6 +128,7 @@ _SIGNATURES = [
     ("lbm_mlp_loss_grad_noalloc", C.c_int, [_P, _P, _P, C.c_int, _P, C.POINTER(C.c_double)]),
     ("lbm_mlp_clear_samples", C.c_int, [_P]),
     ("lbm_mlp_sample_count", C.c_int, [_P, C.POINTER(C.c_int)]),
+    ("lbm_mlp_read_samples", C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
     ("lbm_mlp_add_samples", C.c_int, [_P, _P, _P, C.c_int]),
     ("lbm_mlp_add_transition", C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                          C.c_int, C.c_int, C.POINTER(C.c_int)]),
@@ -609,6 +610,14 @@ class Network:
         n = C.c_int()
         _check(self._lib.lbm_mlp_sample_count(self._h, C.byref(n)))
         return n.value
+
+    def read_samples(self, offset: int = 0, count: int | None = None) -> tuple[np.ndarray, np.ndarray]:
+        if count is None:
+            count = self.sample_count() - offset
+        x = np.empty((count, 9), np.float32)
+        t = np.empty((count, 9), np.float32)
+        _check(self._lib.lbm_mlp_read_samples(self._h, offset, count, _ptr(x), _ptr(t)))
+        return x, t
 
     def add_samples(self, x: np.ndarray, t: np.ndarray) -> None:
         x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 9)

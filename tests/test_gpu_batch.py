@@ -130,7 +130,18 @@ def test_c4_512_domains_sampled_against_single():
         fl = ~d.solid
         assert np.array_equal(got[:, fl], ref[:, fl], equal_nan=True), f"domain {k} differs"
     # plain BGK at tau near 0.51 with inlet speeds up to 0.08 is not stable
-    # on every seeded domain; count the ones that stayed finite
-    finite = sum(bool(np.isfinite(b.reduce(k)).all()) for k in range(len(doms)))
-    print(f"C4: {finite} of {len(doms)} domains finite after 200 steps", flush=True)
-    assert finite > len(doms) // 2
+    # on every seeded domain; name the ones that did not stay finite
+    diverged = [k for k in range(len(doms)) if not np.isfinite(b.reduce(k)).all()]
+    for k in diverged:
+        d = doms[k]
+        print(f"C4 diverged: domain {k} {d.nx}x{d.ny} tau {d.tau:.4f} ux {d.inlet_ux:.4f} "
+              f"disks {len(d.disks)} cell Re {d.inlet_ux / ((d.tau - 0.5) / 3):.1f}", flush=True)
+    print(f"C4: {len(doms) - len(diverged)} of {len(doms)} domains finite after 200 steps: diverged {diverged}",
+          flush=True)
+    # The seeded set: every domain with tau <= 0.5142 (10 of 10; nu <= 0.0047)
+    # and two more at 0.5172 and 0.5215 (large, with 6 and 7 disks) of the
+    # 8 between 0.5143 and 0.5215; none of the 494 above.  Plain BGK this
+    # close to tau = 1/2 does not survive the impulsive start of the Zou-He
+    # inlet on these geometries.
+    assert diverged == [3, 46, 66, 123, 151, 182, 187, 223, 284, 318, 356, 493]
+    assert all(doms[k].tau < 0.5216 for k in diverged)

@@ -270,11 +270,11 @@ def test_restriction_keeps_block_mass_and_momentum():
     cf = ~sc
     assert np.array_equal(sc, nfl < 8)
     m = blk(np.where(fl, rho, 0.0))
-    np.testing.assert_allclose(fc.sum(axis=0)[cf], (m / nfl)[cf], rtol=2e-6)
-    np.testing.assert_allclose(uxc[cf], (blk(np.where(fl, rho * ux, 0.0)) / m)[cf], rtol=1e-4, atol=1e-9)
+    np.testing.assert_allclose(fc.sum(axis=0)[cf], m[cf] / nfl[cf], rtol=2e-6)
+    np.testing.assert_allclose(uxc[cf], blk(np.where(fl, rho * ux, 0.0))[cf] / m[cf], rtol=1e-4, atol=1e-9)
     jx = sum(W.EX[i] * fc[i] for i in range(9))
     # the coarse Guo velocity (sum e f + g_c/2)/rho with g_c = 4 g equals u_c
-    np.testing.assert_allclose(((jx + 2 * g[0]) / fc.sum(axis=0))[cf], uxc[cf], rtol=1e-4, atol=1e-8)
+    np.testing.assert_allclose((jx[cf] + 2 * g[0]) / fc.sum(axis=0)[cf], uxc[cf], rtol=1e-4, atol=1e-8)
     assert np.all(fc[:, sc] == 0) and np.all(uxc[sc] == 0)
 
 
@@ -426,8 +426,9 @@ def test_learned_step_matches_fp64_oracle(kind):
     """k_phys_step_learned (binary32 network) and k_phys_step_learned_mfma
     (bf16 operands, binary32 accumulation) against the binary64 oracle of
     the learned step, on lattices with walls, after 1 and 20 steps from a
-    perturbed state.  Bounds: binary32, |f - f_oracle| <= 2e-7 on every
-    fluid population (f ~ 0.1; the correction itself changes f by ~1e-4);
+    perturbed state.  Bounds: binary32, |f - f_oracle| <= 4e-7 on every
+    fluid population (13 ulp of f_0 ~ 0.44 after 20 steps, measured up to
+    1.7e-7; the correction itself changes f by ~1e-5);
     bf16, the difference from the oracle <= 5 % of what the correction
     changes (rms over fluid populations), the rounding of the network's
     operands to 8 significant bits."""
@@ -456,7 +457,7 @@ def test_learned_step_matches_fp64_oracle(kind):
         errbf = np.sqrt(np.mean((bf[:, fl] - ref[:, fl]) ** 2)) / np.sqrt(np.mean(effect**2))
         print(f"f32 max err {err32:.3e}; bf16 rms err / correction {errbf:.3e}; "
               f"correction rms {np.sqrt(np.mean(effect**2)):.3e}", flush=True)
-        assert err32 <= 2e-7
+        assert err32 <= 4e-7
         assert errbf <= 5e-2
 
 
@@ -547,3 +548,85 @@ def test_learned_handle_exports_macro(storage, precision):
     assert np.array_equal(t1, t2)
     assert np.array_equal(t1[..., 3] == 0, solid)
     lat.close()
+
+
+def test_network_update_after_attachment_reaches_graph_chunks():
+    """Scales and parameters changed after the network is attached (and
+    after the handle captured its 100-step graphs) take effect in later
+    graph chunks: 100 steps, new scales, 200 steps, new parameters, 200
+    steps through graphs equal the same sequence launched step by step."""
+    print("case: network updates reach captured graphs", flush=True)
+    nx, ny = 96, 64
+    solid, kw = walled_scene("periodic", nx, ny)
+    f0 = W.rest_state(nx, ny, solid, amplitude=1e-3, seed=12).astype(np.float32)
+    out = []
+    for chunk in (100, 0):
+        net = L.Network(seed=6)
+        net.set_params(random_params(10, 0.2))
+        net.set_scales(3e-2, 0.05)
+        lat = L.Lattice(nx, ny, 0.56, **kw)
+        lat.write_materials(W.materials_from_solid(solid))
+        lat.write_populations(f0, 0)
+        lat.set_graph_chunk(chunk)
+        lat.set_learned_collision(net)
+        lat.run(100)
+        net.set_scales(3e-2, 0.2)
+        lat.run(200)
+        net.set_params(random_params(11, 0.2))
+        lat.run(200)
+        out.append(lat.read_populations())
+        lat.close()
+        net.close()
+    fl = ~solid
+    assert np.array_equal(out[0][:, fl], out[1][:, fl])
+
+
+def test_transition_samples_match_fp64_oracle():
+    """lbm_mlp_add_transition against the oracle's targets on the same
+    binary32 states: a periodic coarse lattice with a solid disk whose
+    second state comes from the first by a learned step with constant
+    modulations, so the residual is the correction.  At tau = 0.6 (bound
+    b = 1/6) the residual moments are ~1e-4 against populations of ~0.1, so
+    the binary32 subtraction keeps them to ~1e-4 relative (at tau = 0.505,
+    b = 0.01, they shrink to ~1e-6 and the subtraction's cancellation alone
+    is a 1 % error).  Inputs within 1e-5 of their scale; targets within
+    1e-2 rms and 5e-2 max of theirs; counts and order equal."""
+    print("case: transition samples vs fp64 oracle", flush=True)
+    nx, ny = 48, 40
+    tau = 0.6
+    force = (4e-6, 0.0)
+    b = O.modulation_bound(1 / tau)
+    alpha = np.array([0.5 * b, -0.7 * b, 0.8 * b])
+    s_in, s_out = 2e-3, 0.25
+    solid = np.zeros((ny, nx), bool)
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    solid |= (xx - 20) ** 2 + (yy - 18) ** 2 <= 16
+    rng = np.random.default_rng(8)
+    n = nx * ny
+    eu = lambda ux, uy: W.EX[:, None] * ux[None] + W.EY[:, None] * uy[None]
+    rho = 1 + rng.uniform(-0.01, 0.01, n)
+    ux, uy = rng.uniform(-0.05, 0.05, n), rng.uniform(-0.05, 0.05, n)
+    e = eu(ux, uy)
+    s0 = W.W_PHYS[:, None] * rho * (1 + 3 * e + 4.5 * e**2 - 1.5 * (ux**2 + uy**2)) + rng.uniform(-1e-3, 1e-3, (9, n))
+    s0 = s0.reshape(9, ny, nx)
+    prev = np.stack([np.roll(np.roll(s0[i], -W.EY[i], 0), -W.EX[i], 1) for i in range(9)])
+    p = O.phys_params(nx, ny, tau, force=force)
+    net_c = np.zeros(5385)
+    net_c[5376:5379] = alpha / s_out
+    post, _ = O.learned_step(p, solid, prev, net_c, s_in, s_out)
+    s1 = np.stack([np.roll(np.roll(post[i], W.EY[i], 0), W.EX[i], 1) for i in range(9)])
+    s0f, s1f = s0.astype(np.float32), s1.astype(np.float32)
+    X, T = O.transition_samples(s0f, s1f, solid, tau, force, s_in, s_out, stride=3, offset=2)
+    net = L.Network(seed=1, capacity=10000)
+    net.set_scales(s_in, s_out)
+    added = net.add_transition(s0f, s1f, solid, tau, force, stride=3, offset=2)
+    assert added == len(X) > 300
+    xd, td = net.read_samples()
+    assert np.max(np.abs(xd - X)) <= 1e-5 * np.max(np.abs(X))
+    err = td[:, :3] - T[:, :3]
+    print(f"targets: rms err {np.sqrt(np.mean(err**2)) / np.sqrt(np.mean(T[:, :3]**2)):.3e}, "
+          f"max err {np.max(np.abs(err)) / np.max(np.abs(T[:, :3])):.3e}", flush=True)
+    assert np.sqrt(np.mean(err**2)) <= 1e-2 * np.sqrt(np.mean(T[:, :3] ** 2))
+    assert np.max(np.abs(err)) <= 5e-2 * np.max(np.abs(T[:, :3]))
+    assert np.all(td[:, 3:] == 0)
+    net.close()
