@@ -198,24 +198,21 @@ __device__ inline void phys_cell(const PhysArgs& a, const T* __restrict__ src, T
         macro[c + 2 * n] = rho;
         macro[c + 3 * n] = 1.0f;
     }
-    float corr[9];
+    float nneq[3], y9[9];
     bool learned = false;
     if constexpr (LEARNED) {
         // The network is trained on cells whose 3x3 neighbourhood is fluid
         // (away from walls and open edges); only those get the correction.
         learned = (nb.sm | nb.s0 | nb.sp) == 0u && (XEDGE == 0 || (x > 0 && x < a.nx - 1));
         if (learned) {
-            float xin[9], y9[9];
-            cell_features(g, rho, ux, uy, s_in, xin);
+            float inv_ws[3], xin[9];
+            learned_input_scales(s_in, inv_ws);
+            learned_inputs(g, rho, ux, uy, inv_ws, xin, nneq);
             mlp_forward_one(mlp, xin, y9);
-            learned_correction(xin, y9, s_in, s_out, a.omega, corr);
         }
     }
     phys_collide<FORCE, SC>(a, g, rho, ux, uy);
-    if (learned) {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) g[i] += corr[i];
-    }
+    if (learned) add_stress_correction(nneq, y9, s_out, modulation_bound(a.omega), g);
     cell_store<T, XEDGE, FLAGS, SC>(a, dst, nb, x, y, g);
 }
 

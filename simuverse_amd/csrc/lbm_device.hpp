@@ -274,21 +274,21 @@ __device__ inline void phys_moments(const PhysArgs& a, const float g[9], float& 
     float dr = g[0];
 #pragma unroll
     for (int i = 1; i < 9; ++i) dr += g[i];
-    float jx = (g[1] - g[3]) + (g[5] - g[6]) + (g[8] - g[7]);
-    float jy = (g[4] - g[2]) + (g[7] - g[5]) + (g[8] - g[6]);
-    if (SC) {
-        dr *= a.inv_scale;
-        jx *= a.inv_scale;
-        jy *= a.inv_scale;
-    }
+    const float jx = (g[1] - g[3]) + (g[5] - g[6]) + (g[8] - g[7]);
+    const float jy = (g[4] - g[2]) + (g[7] - g[5]) + (g[8] - g[6]);
+    if (SC) dr *= a.inv_scale;
     rho = 1.0f + dr;
     const float ir = __builtin_amdgcn_rcpf(rho);  // hardware reciprocal, 1 ulp
+    // SC: the momentum sums carry the scale, which folds into the operation
+    // that uses them (a product with a power of two is exact, so the
+    // results are those of rescaling the sums first).
     if (FORCE) {
-        ux = (jx + 0.5f * a.gx) * ir;
-        uy = (jy + 0.5f * a.gy) * ir;
+        ux = (SC ? fmaf(jx, a.inv_scale, 0.5f * a.gx) : jx + 0.5f * a.gx) * ir;
+        uy = (SC ? fmaf(jy, a.inv_scale, 0.5f * a.gy) : jy + 0.5f * a.gy) * ir;
     } else {
-        ux = jx * ir;
-        uy = jy * ir;
+        const float irs = SC ? ir * a.inv_scale : ir;
+        ux = jx * irs;
+        uy = jy * irs;
     }
 }
 
@@ -303,13 +303,19 @@ __device__ __forceinline__ float e_dot_u(int i, float ux, float uy)
 
 // BGK (+ Guo forcing) in deviation form:
 //   g*_i = (1 - omega) g_i + omega geq_i + F_i
-// with omega geq_i = omega w_k (rho - 1) + omega w_k rho q_i,
+// with omega geq_i = omega w_k (rho q_i + rho - 1),
 // q_i = e.u (3 + 4.5 e.u) - 1.5 u^2, and F_i = fa_i + fb_i (e_i.u) - fc_k (u.g)
-// (PhysArgs).  Opposite directions share s = fb_i (e_i.u) - fc_k (u.g)
-// (fa and fb flip sign with e_i, and so does e_i.u), so F_i = s + fa_i and
-// F_inv(i) = s - fa_i.  About five fused multiply-adds per direction.
+// (PhysArgs).  Opposite directions i, inv(i) have opposite e.u, so with
+//   s = 4.5 (e.u)^2 - 1.5 u^2,  c = rho s + rho - 1,  d = 3 rho e.u
+// their rho q + rho - 1 are c + d and c - d, and their forcing terms
+// A + fa_i and A - fa_i with A = fb_i (e_i.u) - fc_k (u.g) (fa and fb flip
+// sign with e_i):
+//   g*_i = (1 - omega) g_i + (S + D),  g*_inv(i) = (1 - omega) g_inv(i) + (S - D),
+//   S = omega w_k c + A,  D = omega w_k d + fa_i,
+// about eleven operations per pair of directions.
 // SC: g holds scaled deviations and the result is scaled (the constants
-// sow, sfa, sfb, sfc carry the scale).
+// sow, sfa, sfb, sfc carry the scale; every sum and product is the unscaled
+// one times a power of two).
 template <bool FORCE, bool SC = false>
 __device__ inline void phys_collide(const PhysArgs& a, float g[9], float rho, float ux, float uy)
 {
@@ -319,46 +325,77 @@ __device__ inline void phys_collide(const PhysArgs& a, float g[9], float rho, fl
     const float* const fc = SC ? a.sfc : a.fc;
     const float dr = rho - 1.0f;
     const float usq = 1.5f * fmaf(ux, ux, uy * uy);
-    const float owr[3] = {ow[0] * rho, ow[1] * rho, ow[2] * rho};
-    const float owd[3] = {ow[0] * dr, ow[1] * dr, ow[2] * dr};
-    float F[9];
+    const float r3 = 3.0f * rho;
+    float fu[3] = {0.0f, 0.0f, 0.0f};
     if (FORCE) {
         const float ug = fmaf(ux, a.gx, uy * a.gy);
-        const float fu[3] = {-fc[0] * ug, -fc[1] * ug, -fc[2] * ug};
-        F[0] = fu[0];  // e_0 = 0: fa_0 = fb_0 = 0
-#pragma unroll
-        for (int i = 1; i < 9; ++i) {
-            if (inv_of(i) < i) continue;  // each pair once, from its lower index
-            const float sp = fmaf(fb[i], e_dot_u(i, ux, uy), fu[wclass(i)]);
-            F[i] = sp + fa[i];
-            F[inv_of(i)] = sp - fa[i];
-        }
+        fu[0] = -fc[0] * ug;
+        fu[1] = -fc[1] * ug;
+        fu[2] = -fc[2] * ug;
     }
+    {
+        // rest direction: e = 0, so rho q_0 + rho - 1 = dr - rho 1.5 u^2
+        const float c0 = fmaf(-rho, usq, dr);
+        const float S0 = FORCE ? fmaf(ow[0], c0, fu[0]) : ow[0] * c0;
+        g[0] = fmaf(g[0], a.rom, S0);
+    }
+#pragma unroll
+    for (int i = 1; i < 9; ++i) {
+        const int j = inv_of(i);
+        if (j < i) continue;  // each pair once, from its lower index
+        const int k = wclass(i);
+        const float eu = e_dot_u(i, ux, uy);
+        const float s = fmaf(4.5f * eu, eu, -usq);
+        const float c = fmaf(rho, s, dr);
+        const float d = r3 * eu;
+        const float S = FORCE ? fmaf(ow[k], c, fmaf(fb[i], eu, fu[k])) : ow[k] * c;
+        const float D = FORCE ? fmaf(ow[k], d, fa[i]) : ow[k] * d;
+        g[i] = fmaf(g[i], a.rom, S + D);
+        g[j] = fmaf(g[j], a.rom, S - D);
+    }
+}
+
+// The learned correction's inputs and the moments it modulates, from the
+// gathered deviations g and their moments rho, u:
+//   q_i = g_i - geq_i = f_i - feq_i,
+//   geq_i = w_i (rho - 1) + w_i rho (e_i.u (3 + 4.5 e_i.u) - 1.5 u^2),
+//   x_i = q_i / (w_i s_in)                      (the network's input)
+//   n_e  = sum m_e(i) q_i  = -4 q_0 - (q_1 + q_2 + q_3 + q_4) + 2 (q_5 + .. + q_8)
+//   n_xx = sum m_xx(i) q_i = (q_1 + q_3) - (q_2 + q_4)
+//   n_xy = sum m_xy(i) q_i = (q_6 + q_8) - (q_5 + q_7)
+// (moment rows as in add_stress_correction).  inv_ws[k] = 1 / (w_k s_in) for
+// the rest, axis and diagonal weight classes; two fused operations per
+// direction for the equilibrium.
+__host__ __device__ constexpr float inv_w_phys(int i) { return i == 0 ? 2.25f : (i <= 4 ? 9.0f : 36.0f); }
+__device__ __forceinline__ void learned_inputs(const float g[9], float rho, float ux, float uy,
+                                               const float inv_ws[3], float xin[9], float n[3])
+{
+    const float dr = rho - 1.0f;
+    const float usq = 1.5f * fmaf(ux, ux, uy * uy);
+    const float wr[3] = {w_phys(0) * rho, w_phys(1) * rho, w_phys(5) * rho};
+    const float wd[3] = {w_phys(0) * dr, w_phys(1) * dr, w_phys(5) * dr};
+    float q[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const int k = wclass(i);
         const float eu = e_dot_u(i, ux, uy);
-        const float q = i == 0 ? -usq : fmaf(eu, fmaf(4.5f, eu, 3.0f), -usq);
-        const float t = FORCE ? fmaf(g[i], a.rom, F[i]) : g[i] * a.rom;
-        g[i] = fmaf(owr[k], q, owd[k] + t);
+        const float t = i == 0 ? -usq : fmaf(eu, fmaf(4.5f, eu, 3.0f), -usq);
+        q[i] = g[i] - fmaf(wr[k], t, wd[k]);
+        xin[i] = q[i] * inv_ws[k];
     }
+    const float ax = q[1] + q[3], ay = q[2] + q[4];
+    const float d57 = q[5] + q[7], d68 = q[6] + q[8];
+    n[0] = fmaf(2.0f, d57 + d68, fmaf(-4.0f, q[0], -(ax + ay)));
+    n[1] = ax - ay;
+    n[2] = d68 - d57;
 }
-
-// The learned correction's input: the non-equilibrium part g - geq = f - feq
-// over w_i s_in, as a product with the reciprocal (1/w_i is exact: 9/4, 9,
-// 36; one division per cell instead of nine).
-__host__ __device__ constexpr float inv_w_phys(int i) { return i == 0 ? 2.25f : (i <= 4 ? 9.0f : 36.0f); }
-__device__ __forceinline__ void cell_features(const float g[9], float rho, float ux, float uy, float s_in,
-                                              float xin[9])
+// inv_ws of learned_inputs for the scale s_in.
+__host__ __device__ inline void learned_input_scales(float s_in, float inv_ws[3])
 {
-    const float dr = rho - 1.0f, usq = 1.5f * (ux * ux + uy * uy);
     const float inv_s = 1.0f / s_in;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const float eu = (float)ex_of(i) * ux + (float)ey_of(i) * uy;
-        const float geq = w_phys(i) * (dr + rho * (3.0f * eu + 4.5f * (eu * eu) - usq));
-        xin[i] = (g[i] - geq) * (inv_w_phys(i) * inv_s);
-    }
+    inv_ws[0] = inv_w_phys(0) * inv_s;
+    inv_ws[1] = inv_w_phys(1) * inv_s;
+    inv_ws[2] = inv_w_phys(5) * inv_s;
 }
 
 // Arguments of the parity-mode (reference semantics) kernels.
@@ -553,7 +590,10 @@ __device__ __forceinline__ void mlp_wave_bf16(const __bf16* __restrict__ W, cons
     const float* b1 = bias;
     const float* b2 = bias + kMlpH;
     const float* b3 = bias + 2 * kMlpH;
-#pragma unroll
+    // The two sample tiles one after the other: fewer live accumulators,
+    // and the other wavefronts of the SIMD fill the matrix-instruction
+    // latency (2 % faster than interleaving them in one wavefront).
+#pragma unroll 1
     for (int nt = 0; nt < 2; ++nt) {
         const bf16x8 xb = lds_b128(xs + (32 * nt + n) * 16 + 8 * h);
         bf16x8 hb[4];
@@ -677,42 +717,29 @@ __host__ __device__ inline float modulation_hi(float omega) { return modulation_
 // mapped back with the orthogonal basis (project_stress):
 //   corr_i = sum_a m_a(i) dm_a / |m_a|^2,
 // so the correction carries no mass, no momentum and no ghost moments.
-// xin holds the network inputs (f_i - feq_i) / (w_i s_in).  The two halves
-// are separate so a kernel can form n_a before the network runs and keep
-// three values, not nine, live across it.
-__device__ inline void stress_neq(const float xin[9], float s_in, float n[3])
+// learned_inputs forms n_a before the network runs, so a kernel keeps three
+// values, not nine, live across it.  With the rows written out:
+//   corr_0 = -4 c_e,  corr_1,3 = c_xx - c_e,  corr_2,4 = -c_xx - c_e,
+//   corr_5,7 = 2 c_e - c_xy,  corr_6,8 = 2 c_e + c_xy
+// (c_e = alpha_e n_e / 36, c_xx = alpha_xx n_xx / 4, c_xy = alpha_xy n_xy / 4),
+// added to g.  b is modulation_bound(omega).
+__device__ __forceinline__ void add_stress_correction(const float n[3], const float y[3], float s_out, float b,
+                                                      float g[9])
 {
-    float ne = 0.0f, nxx = 0.0f, nxy = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const float ex = (float)ex_of(i), ey = (float)ey_of(i);
-        const float q = xin[i] * (w_phys(i) * s_in);
-        ne += mom_e(i) * q;
-        nxx += (ex * ex - ey * ey) * q;
-        nxy += (ex * ey) * q;
-    }
-    n[0] = ne;
-    n[1] = nxx;
-    n[2] = nxy;
-}
-__device__ inline void stress_correction(const float n[3], const float y[3], float s_out, float omega, float corr[9])
-{
-    const float lo = modulation_lo(omega), hi = modulation_hi(omega);
-    const float ce = fminf(fmaxf(s_out * y[0], lo), hi) * n[0] * (1.0f / 36.0f);
-    const float cxx = fminf(fmaxf(s_out * y[1], lo), hi) * n[1] * 0.25f;
-    const float cxy = fminf(fmaxf(s_out * y[2], lo), hi) * n[2] * 0.25f;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const float ex = (float)ex_of(i), ey = (float)ey_of(i);
-        corr[i] = mom_e(i) * ce + (ex * ex - ey * ey) * cxx + (ex * ey) * cxy;
-    }
-}
-__device__ inline void learned_correction(const float xin[9], const float y[9], float s_in, float s_out, float omega,
-                                          float corr[9])
-{
-    float n[3];
-    stress_neq(xin, s_in, n);
-    stress_correction(n, y, s_out, omega, corr);
+    const float ce = fminf(fmaxf(s_out * y[0], -b), b) * n[0] * (1.0f / 36.0f);
+    const float cxx = fminf(fmaxf(s_out * y[1], -b), b) * n[1] * 0.25f;
+    const float cxy = fminf(fmaxf(s_out * y[2], -b), b) * n[2] * 0.25f;
+    const float ax = cxx - ce, ay = -cxx - ce;
+    const float d57 = fmaf(2.0f, ce, -cxy), d68 = fmaf(2.0f, ce, cxy);
+    g[0] = fmaf(-4.0f, ce, g[0]);
+    g[1] += ax;
+    g[3] += ax;
+    g[2] += ay;
+    g[4] += ay;
+    g[5] += d57;
+    g[7] += d57;
+    g[6] += d68;
+    g[8] += d68;
 }
 
 // Linear block id -> logical tile id such that the blocks the hardware

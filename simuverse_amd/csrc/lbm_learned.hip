@@ -164,34 +164,31 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t total = nseg * (uint32_t)a.ny;
+    float inv_ws[3];
+    learned_input_scales(s_in, inv_ws);
+    const float bound = modulation_bound(a.omega);
     for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
         const int y = (int)(tile / nseg);
         const int x = (int)(tile % nseg) * 256 + (int)threadIdx.x;
-        bool fluid = false, learned = false;
-        CellNbr nb{};
-        float g[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        float rho = 1.0f, ux = 0.0f, uy = 0.0f;
-        if (x < a.nx) {
-            nb = cell_nbr(a, bits, words, x, y);
-            fluid = !(nb.s0 & 2u);
-            // trained on cells whose 3x3 neighbourhood is fluid: only those
-            // get the correction
-            learned = (nb.sm | nb.s0 | nb.sp) == 0u && (XEDGE == 0 || (x > 0 && x < a.nx - 1));
-        }
-        float xin[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        float nneq[3] = {0.0f, 0.0f, 0.0f};
-        if (fluid) {
-            cell_gather<T, XEDGE, kNtStore>(a, src, nb, x, y, g);
-            phys_moments<FORCE>(a, g, rho, ux, uy);
-            if (learned) {
-                cell_features(g, rho, ux, uy, s_in, xin);
-                stress_neq(xin, s_in, nneq);
-            }
-            // BGK now, the correction after the network: only g, the three
-            // moments and the neighbour bits stay live across it.
-            phys_collide<FORCE>(a, g, rho, ux, uy);
-        }
-        if (macro && x < a.nx) {
+        // Every lane runs the same straight-line arithmetic up to the
+        // network: a lane past the row works on the row's last cell, a lane
+        // on a solid cell on whatever its slots hold, and neither stores
+        // anything nor feeds the network (the wavefront takes no branch and
+        // sets no registers to zero for them).
+        const bool in_row = x < a.nx;
+        const int xc = in_row ? x : a.nx - 1;
+        const CellNbr nb = cell_nbr(a, bits, words, xc, y);
+        const bool fluid = in_row && !(nb.s0 & 2u);
+        // trained on cells whose 3x3 neighbourhood is fluid: only those get
+        // the correction
+        const bool learned =
+            in_row && (nb.sm | nb.s0 | nb.sp) == 0u && (XEDGE == 0 || (x > 0 && x < a.nx - 1));
+        float g[9], rho, ux, uy;
+        cell_gather<T, XEDGE, kNtStore>(a, src, nb, xc, y, g);
+        phys_moments<FORCE>(a, g, rho, ux, uy);
+        float xin[9], nneq[3];
+        learned_inputs(g, rho, ux, uy, inv_ws, xin, nneq);
+        if (macro && in_row) {
             // the macro field of the gathered populations (0 on solids)
             const uint32_t n = (uint32_t)a.nx * (uint32_t)a.ny, c = (uint32_t)y * (uint32_t)a.nx + (uint32_t)x;
             macro[c] = fluid ? ux : 0.0f;
@@ -199,11 +196,20 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
             macro[c + 2 * n] = fluid ? rho : 0.0f;
             macro[c + 3 * n] = fluid ? 1.0f : 0.0f;
         }
+        // BGK now, the correction after the network: only g, the three
+        // moments and the neighbour bits stay live across it.
+        phys_collide<FORCE>(a, g, rho, ux, uy);
         bf16x8 lo, hi;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             lo[i] = (__bf16)xin[i];
             hi[i] = (__bf16)(i == 0 ? xin[8] : 0.0f);
+        }
+        // cells that are not corrected feed zeros (their column of the
+        // product is ignored anyway; zeros keep it finite)
+        if (!learned) {
+            lo = bf16x8{};
+            hi = bf16x8{};
         }
         *reinterpret_cast<bf16x8*>(&xs[wave][lane * 16]) = lo;
         *reinterpret_cast<bf16x8*>(&xs[wave][lane * 16 + 8]) = hi;
@@ -216,17 +222,13 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
         mlp_wave_bf16(W, bias, xs[wave], ys[wave], lane);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        if (fluid) {
-            if (learned) {
-                float corr[9], y3[3];
+        if (learned) {
+            float y3[3];
 #pragma unroll
-                for (int i = 0; i < 3; ++i) y3[i] = ys[wave][lane * kMlpOut + i];
-                stress_correction(nneq, y3, s_out, a.omega, corr);
-#pragma unroll
-                for (int i = 0; i < 9; ++i) g[i] += corr[i];
-            }
-            cell_store<T, XEDGE, kNtStore>(a, dst, nb, x, y, g);
+            for (int i = 0; i < 3; ++i) y3[i] = ys[wave][lane * kMlpOut + i];
+            add_stress_correction(nneq, y3, s_out, bound, g);
         }
+        if (fluid) cell_store<T, XEDGE, kNtStore>(a, dst, nb, x, y, g);
     }
 }
 

@@ -370,8 +370,9 @@ __global__ __launch_bounds__(256) void k_mlp_transition(const float* __restrict_
     for (int i = 0; i < 9; ++i) g[i] = s0[(size_t)i * n + c] - w_phys(i);
     float rho, ux, uy;
     phys_moments<true>(a, g, rho, ux, uy);
-    float xin[9], d[9];
-    cell_features(g, rho, ux, uy, s_in, xin);
+    float xin[9], d[9], nm[3], inv_ws[3];
+    learned_input_scales(s_in, inv_ws);
+    learned_inputs(g, rho, ux, uy, inv_ws, xin, nm);
 #pragma unroll
     for (int i = 0; i < 9; ++i) gc[i] = g[i];
     phys_collide<true>(a, gc, rho, ux, uy);
@@ -380,26 +381,23 @@ __global__ __launch_bounds__(256) void k_mlp_transition(const float* __restrict_
         const int tx = (x + ex_of(i) + nx) % nx, ty = (y + ey_of(i) + ny) % ny;
         d[i] = (s1[(size_t)i * n + (uint32_t)ty * nx + tx] - w_phys(i)) - gc[i];
     }
-    // Targets of the three modulations (learned_correction): residual
+    // Targets of the three modulations (add_stress_correction): residual
     // moment over non-equilibrium moment, d_a / n_a, regularised where n_a
     // is small against the cell's stress: d_a n_a / (n_a^2 + eps^2) with
     // eps^2 = 1e-2 (n_xx^2 + n_xy^2 + n_e^2 / 9), clamped to the bounds the
-    // step applies.
-    float ne = 0.0f, nxx = 0.0f, nxy = 0.0f, de = 0.0f, dxx = 0.0f, dxy = 0.0f;
+    // step applies.  nm holds n_e, n_xx, n_xy (learned_inputs).
+    float de = 0.0f, dxx = 0.0f, dxy = 0.0f;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         const float ex = (float)ex_of(i), ey = (float)ey_of(i);
-        const float q = xin[i] * (w_phys(i) * s_in);
-        ne += mom_e(i) * q;
-        nxx += (ex * ex - ey * ey) * q;
-        nxy += (ex * ey) * q;
         de += mom_e(i) * d[i];
         dxx += (ex * ex - ey * ey) * d[i];
         dxy += (ex * ey) * d[i];
     }
+    const float ne = nm[0], nxx = nm[1], nxy = nm[2];
     const float eps2 = 1e-2f * (nxx * nxx + nxy * nxy + ne * ne * (1.0f / 9.0f)) + 1e-30f;
     const float lo = modulation_lo(a.omega), hi = modulation_hi(a.omega);
-    auto ratio = [&](float dm, float nm) { return fminf(fmaxf(dm * nm / (nm * nm + eps2), lo), hi); };
+    auto ratio = [&](float dm, float n_a) { return fminf(fmaxf(dm * n_a / (n_a * n_a + eps2), lo), hi); };
     float t[9] = {ratio(de, ne), ratio(dxx, nxx), ratio(dxy, nxy), 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     const int slot = block_base[blockIdx.x] + rank;
     if (slot >= max_samples) return;
