@@ -152,10 +152,14 @@ def cpu_baseline(scene, budget_s: float = 12.0):
 
 def describe_variant(shape):
     """The step-kernel variant as a dict: one-step shapes (block,
-    cells_per_thread, vec) or the two-step LDS tile (rows x width)."""
+    cells_per_thread, vec), the two-step LDS tile (rows x width) or the
+    two-step row stream (strips of block columns, chunks of rows)."""
     b, c, v, f = shape
     flags = [name for bit, name in ((1, "nt_store"), (2, "nt_load"), (4, "xcd_rows"), (8, "lane_shuffle"),
-                                    (16, "two_step")) if f & bit]
+                                    (16, "two_step"), (128, "row_stream")) if f & bit]
+    if f & 128:
+        return {"kernel": "k_phys_step2_rows", "block": b, "chunk_rows": c, "strip_width": v, "flags": flags,
+                "steps_per_launch": 2}
     if f & 16:
         return {"kernel": "k_phys_step2", "block": b, "tile_rows": c, "tile_width": v, "flags": flags,
                 "steps_per_launch": 2}

@@ -138,18 +138,15 @@ __device__ __forceinline__ void cell_gather(const PhysArgs& a, const T* __restri
 // slots, and pushes f_i into slot inv(i) of each solid neighbour x + e_i
 // (the mask's edge bits are empty for a non-periodic x, so nothing is pushed
 // outside).
-template <typename T, int XEDGE, int FLAGS, bool SC = false>
-__device__ __forceinline__ void cell_store(const PhysArgs& a, T* __restrict__ dst, const CellNbr& nb, int x, int y,
-                                           const float g[9])
+// The pushes of cell_store alone: f_i into slot inv(i) of each solid
+// neighbour x + e_i of the fluid cell (x, y).
+template <typename T, int XEDGE, bool SC = false>
+__device__ __forceinline__ void cell_push(const PhysArgs& a, T* __restrict__ dst, const CellNbr& nb, int x, int y,
+                                          const float g[9])
 {
+    if ((nb.sm | nb.sp | (nb.s0 & 5u)) == 0u) return;
     const int nx = a.nx;
     const uint32_t r0 = (uint32_t)y * a.rs;
-    // one per-lane pointer; the direction offsets are uniform across the wave
-    T* const own = dst + (r0 + (uint32_t)x);
-#pragma unroll
-    for (int i = 0; i < 9; ++i)
-        st<(FLAGS & kNtStore) != 0>(own + (size_t)((uint32_t)i * a.plane), Dom<T, SC>::out(g[i], a.scale));
-    if ((nb.sm | nb.sp | (nb.s0 & 5u)) == 0u) return;
     int xm = x - 1, xp = x + 1;
     if (XEDGE == 0) {
         if (xm < 0) xm = nx - 1;
@@ -165,6 +162,18 @@ __device__ __forceinline__ void cell_store(const PhysArgs& a, T* __restrict__ ds
         const uint32_t trow = ey == 1 ? rp : (ey == -1 ? rm : r0);
         Dom<T, SC>::push(dst, trow + (uint32_t)tx + (uint32_t)inv_of(i) * a.plane, g[i], w_phys(i), a.scale);
     }
+}
+
+template <typename T, int XEDGE, int FLAGS, bool SC = false>
+__device__ __forceinline__ void cell_store(const PhysArgs& a, T* __restrict__ dst, const CellNbr& nb, int x, int y,
+                                           const float g[9])
+{
+    // one per-lane pointer; the direction offsets are uniform across the wave
+    T* const own = dst + ((uint32_t)y * a.rs + (uint32_t)x);
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+        st<(FLAGS & kNtStore) != 0>(own + (size_t)((uint32_t)i * a.plane), Dom<T, SC>::out(g[i], a.scale));
+    cell_push<T, XEDGE, SC>(a, dst, nb, x, y, g);
 }
 
 // LEARNED adds the network's projected correction to the collision; mlp

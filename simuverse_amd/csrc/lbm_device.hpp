@@ -87,7 +87,8 @@ enum : int {
     kXcdRemap = 4, // give each XCD a contiguous range of rows
     kShfl = 8,     // aligned vector loads, x-neighbours from the adjacent lane
     kTwoStep = 16, // two steps per launch through an LDS tile (temporal blocking)
-    kGridStride = 32  // batched: block b takes chunks b, b + grid, ... (else a contiguous range)
+    kGridStride = 32, // batched: block b takes chunks b, b + grid, ... (else a contiguous range)
+    kRowStream = 128  // two-step: strips of block-size columns streamed down a chunk of rows
 };
 
 // Population storage of the physics mode.  Kernels work on deviations

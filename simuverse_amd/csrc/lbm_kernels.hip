@@ -777,6 +777,14 @@ const StepVariant kStepVariants[] = {
     {1024, 16, 64, kNtStore | kTwoStep | kXcdRemap},
     {256, 8, 128, kNtStore | kTwoStep | kXcdRemap},
     {512, 32, 64, kNtStore | kTwoStep | kXcdRemap},
+    // two steps per launch streamed down strips: (block, chunk rows, strip width = block, flags)
+    {512, 128, 512, kNtStore | kTwoStep | kRowStream | kXcdRemap},
+    {512, 64, 512, kNtStore | kTwoStep | kRowStream | kXcdRemap},
+    {256, 64, 256, kNtStore | kTwoStep | kRowStream | kXcdRemap},
+    {512, 128, 512, kNtStore | kTwoStep | kRowStream},
+    {1024, 128, 1024, kNtStore | kTwoStep | kRowStream | kXcdRemap},
+    {512, 256, 512, kNtStore | kTwoStep | kRowStream | kXcdRemap},
+    {256, 128, 256, kNtStore | kTwoStep | kRowStream | kXcdRemap},
 };
 const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
 

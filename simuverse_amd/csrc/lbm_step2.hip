@@ -177,6 +177,145 @@ __global__ __launch_bounds__(BS) void k_phys_step2(PhysArgs a, const T* __restri
     two_step_tile<T, XEDGE, FORCE, BS, BW, BH, FLAGS & ~kXcdRemap>(a, src, dst, bits, words, x0, y0, lds);
 }
 
+// Two steps per launch, streamed down a strip of rows.  A block of BS
+// threads owns the columns [x0, x0 + BS) of the rows [y0, y0 + H) (the last
+// strip and chunk may be cut by the lattice edge) and walks down them one
+// row at a time.  Step 1 of region row r -- lattice row y0 - 1 + r, the
+// strip's columns plus one on each side -- gathers from HBM as the tile
+// kernel's step 1 does and leaves its post-collision populations in slot
+// r mod 4 of an LDS ring of four rows; after one barrier, step 2 of lattice
+// row y0 + r - 2 gathers from the slots of region rows r - 2 .. r with the
+// tile kernel's step-2 rules (own opposite population from a solid source
+// or from outside the channel, the outlet gathering as its west neighbour,
+// the Zou-He inlet) and stores to HBM with the pushes into solid
+// neighbours, so the destination ends up exactly as two one-step launches
+// leave it.  Four slots make the one barrier enough: step 1 of row r + 1,
+// which a wave may start while another still runs step 2 of row r - 2,
+// writes the one slot that step reads none of.
+// Against the tile kernel: a step-1 gather or a step-2 store of one wave
+// covers 64 consecutive cells of one direction row (128 B of binary16, a
+// 1 KiB run per block row at 512 threads) instead of cells from two region
+// rows, every direction row of the chunk is read once (the tile re-reads
+// its ring), and the ring adds two cells per row of BS instead of
+// 2 (BW + BH) + 4 per tile.  Step 1's two ring cells are computed by lanes 0
+// and 1 of the first wave.
+template <typename T, int XEDGE, bool FORCE, int BS, int H, int FLAGS>
+__global__ __launch_bounds__(BS) void k_phys_step2_rows(PhysArgs a, const T* __restrict__ src, T* __restrict__ dst,
+                                                        const uint32_t* __restrict__ bits, uint32_t words,
+                                                        uint32_t strips)
+{
+    constexpr int RW = BS + 2;
+    constexpr bool SC = kScaledDom<T>;
+    __shared__ T lds[4][9][RW];
+    const uint32_t t = (FLAGS & kXcdRemap) ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int x0 = (int)(t % strips) * BS, y0 = (int)(t / strips) * H;
+    const int nx = a.nx, ny = a.ny;
+    const int rows = min(H, ny - y0);
+    const int tid = (int)threadIdx.x;
+    // The wave's step-1 cells x0 + 64w .. + 63 and their gathers lie inside
+    // the lattice and away from the inlet and outlet columns (wave-uniform).
+    const int xw = x0 + __builtin_amdgcn_readfirstlane(tid & ~63);
+    const bool wave_inner = xw >= 1 && xw + 65 <= nx;
+
+    // Step 1 of one cell (x, y), unwrapped, into column q of slot `slot`.
+    auto step1_general = [&](int x, int y, int q, int slot) {
+        if (x > nx) return;                              // beyond the ring of a cut strip
+        if (XEDGE == 1 && (x < 0 || x >= nx)) return;    // outside the channel: never read
+        x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+        CellNbr nb;
+        nb.ym = (y == 0) ? ny - 1 : y - 1;
+        nb.yp = (y == ny - 1) ? 0 : y + 1;
+        nb.sm = nb.sp = 0u;
+        nb.s0 = 0u;
+        if (XEDGE == 1) {
+            nb.s0 = solid3(bits, words, y, x);  // the outlet gathers through its west bit
+            if (nb.s0 & 2u) return;
+        }
+        float g[9];
+        cell_gather<T, XEDGE, FLAGS, SC>(a, src, nb, x, y, g);
+        float rho, ux, uy;
+        phys_moments<FORCE, SC>(a, g, rho, ux, uy);
+        phys_collide<FORCE, SC>(a, g, rho, ux, uy);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) lds[slot][i][q] = Dom<T, SC>::out(g[i], a.scale);
+    };
+    auto step1_row = [&](int r) {
+        const int y = y0 - 1 + r, slot = r & 3;
+        const int x = x0 + tid;
+        if (wave_inner && y >= 1 && y + 2 <= ny) {
+            // no wrap, no edge column: one pointer per cell, wave-uniform
+            // direction offsets (periodic x computes solid cells too: step 2
+            // never reads their entries)
+            if (!(XEDGE == 1 && (solid3(bits, words, y, x) & 2u))) {
+                const T* const cell = src + ((uint32_t)y * a.rs + (uint32_t)x);
+                float g[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) {
+                    const ptrdiff_t off = (ptrdiff_t)((uint32_t)i * a.plane) -
+                                          (ptrdiff_t)ey_of(i) * (ptrdiff_t)a.rs - (ptrdiff_t)ex_of(i);
+                    g[i] = Dom<T, SC>::in(ld<(FLAGS & kNtLoad) != 0>(cell + off), a.inv_scale);
+                }
+                float rho, ux, uy;
+                phys_moments<FORCE, SC>(a, g, rho, ux, uy);
+                phys_collide<FORCE, SC>(a, g, rho, ux, uy);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) lds[slot][i][tid + 1] = Dom<T, SC>::out(g[i], a.scale);
+            }
+        } else {
+            step1_general(x, y, tid + 1, slot);
+        }
+        if (tid < 2) step1_general(tid == 0 ? x0 - 1 : x0 + BS, y, tid == 0 ? 0 : BS + 1, slot);
+    };
+    // Step 2 of lattice row y0 + j from the slots of region rows j .. j + 2.
+    auto step2_row = [&](int j) {
+        const int x = x0 + tid, y = y0 + j;
+        if (x >= nx) return;
+        const CellNbr nb = cell_nbr(a, bits, words, x, y);
+        if (nb.s0 & 2u) return;
+        float g[9];
+        const bool outlet = XEDGE == 1 && x == nx - 1 && !(nb.s0 & 1u);
+        if ((nb.sm | nb.sp | (nb.s0 & 5u)) == 0u && !outlet && (XEDGE == 0 || x > 0)) {
+            // no solid neighbour, no edge: plain pulls from the ring
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                g[i] = Dom<T, SC>::in(lds[(j + 1 - ey_of(i)) & 3][i][tid + 1 - ex_of(i)], a.inv_scale);
+        } else {
+            // gathering cell: the outlet column gathers as its west neighbour
+            int cx = x;
+            CellNbr gb = nb;
+            if (outlet) {
+                cx = x - 1;
+                gb = cell_nbr(a, bits, words, cx, y);
+            }
+            const int c = cx - x0 + 1;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                const int ex = ex_of(i), ey = ey_of(i);
+                // source cell cx - e_i: bit (1 - ex) of row y - ey
+                const uint32_t srow = ey == 1 ? gb.sm : (ey == -1 ? gb.sp : gb.s0);
+                bool own = (srow >> (1 - ex)) & 1u;
+                if (XEDGE == 1 && ex != 0) own = own || cx - ex < 0 || cx - ex >= nx;
+                const T v = own ? lds[(j + 1) & 3][inv_of(i)][c] : lds[(j + 1 - ey) & 3][i][c - ex];
+                g[i] = Dom<T, SC>::in(v, a.inv_scale);
+            }
+            if (XEDGE == 1 && x == 0) zou_he_inlet<SC>(a, g);
+        }
+        float rho, ux, uy;
+        phys_moments<FORCE, SC>(a, g, rho, ux, uy);
+        phys_collide<FORCE, SC>(a, g, rho, ux, uy);
+        cell_store<T, XEDGE, FLAGS, SC>(a, dst, nb, x, y, g);
+    };
+
+    step1_row(0);
+    step1_row(1);
+    for (int j = 0; j < rows; ++j) {
+        step1_row(j + 2);
+        __syncthreads();
+        step2_row(j);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Batched domains: one persistent launch steps every domain.
 // ---------------------------------------------------------------------------
@@ -284,6 +423,18 @@ static hipError_t launch_phys_variant2(const PhysArgs& a, const void* src, void*
     return hipGetLastError();
 }
 
+template <typename T, int XEDGE, bool FORCE, int BS, int H, int FLAGS>
+static hipError_t launch_phys_rows(const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
+                                   uint32_t words, hipStream_t s)
+{
+    if (!two_step_fits(BS, a.nx, a.ny, XEDGE)) return hipErrorInvalidValue;
+    const uint32_t strips = (uint32_t)((a.nx + BS - 1) / BS);
+    const uint32_t blocks = strips * (uint32_t)((a.ny + H - 1) / H);
+    hipLaunchKernelGGL((k_phys_step2_rows<T, XEDGE, FORCE, BS, H, FLAGS>), dim3(blocks), dim3(BS), 0, s, a,
+                       (const T*)src, (T*)dst, bits, words, strips);
+    return hipGetLastError();
+}
+
 template <typename T, int XEDGE, bool FORCE>
 static hipError_t dispatch_variant2(int v, const PhysArgs& a, const void* src, void* dst, const uint32_t* bits,
                                     uint32_t words, hipStream_t s)
@@ -306,6 +457,13 @@ static hipError_t dispatch_variant2(int v, const PhysArgs& a, const void* src, v
     case 31: return launch_phys_variant2<T, XEDGE, FORCE, 1024, 64, 16, S | R>(a, src, dst, bits, words, s);
     case 32: return launch_phys_variant2<T, XEDGE, FORCE, 256, 128, 8, S | R>(a, src, dst, bits, words, s);
     case 33: return launch_phys_variant2<T, XEDGE, FORCE, 512, 64, 32, S | R>(a, src, dst, bits, words, s);
+    case 34: return launch_phys_rows<T, XEDGE, FORCE, 512, 128, S | R>(a, src, dst, bits, words, s);
+    case 35: return launch_phys_rows<T, XEDGE, FORCE, 512, 64, S | R>(a, src, dst, bits, words, s);
+    case 36: return launch_phys_rows<T, XEDGE, FORCE, 256, 64, S | R>(a, src, dst, bits, words, s);
+    case 37: return launch_phys_rows<T, XEDGE, FORCE, 512, 128, S>(a, src, dst, bits, words, s);
+    case 38: return launch_phys_rows<T, XEDGE, FORCE, 1024, 128, S | R>(a, src, dst, bits, words, s);
+    case 39: return launch_phys_rows<T, XEDGE, FORCE, 512, 256, S | R>(a, src, dst, bits, words, s);
+    case 40: return launch_phys_rows<T, XEDGE, FORCE, 256, 128, S | R>(a, src, dst, bits, words, s);
     default: return hipErrorInvalidValue;
     }
 }

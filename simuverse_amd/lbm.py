@@ -232,7 +232,8 @@ def variant_shapes() -> list[tuple[int, int, int, int]]:
     flags: 1 non-temporal stores, 2 non-temporal loads, 4 rows per XCD,
     8 aligned vector loads with x-neighbours from the adjacent lane, 16 two
     steps per launch through an LDS tile of cells_per_thread rows x vec
-    cells)."""
+    cells, 64 the tile's step-1 loads all issued first, 128 the two steps
+    streamed down strips of vec columns in chunks of cells_per_thread rows)."""
     lib = load_library()
     out = []
     for v in range(lib.lbm_variant_count()):

@@ -90,8 +90,9 @@ def test_tiny_periodic_lattices_against_oracles(nx, ny):
     one-cell ring wrap around the whole lattice: parity mode bit for bit
     against the WGSL restatement (both buffers, every slot: no solids), and
     physics mode with a body force (and a solid cell when the lattice has
-    room), with the default one-step shape and the 128 x 8 two-step shape,
-    against the binary64 oracle; after 1, 2, 7 and 100 steps."""
+    room), with the default one-step shape, the 128 x 8 two-step tile and
+    the 512-column two-step row stream, against the binary64 oracle; after
+    1, 2, 7 and 100 steps."""
     from test_oracle_wgsl import bulk_info
     print(f"case: tiny lattice {nx}x{ny}", flush=True)
     rng = np.random.default_rng(100 * nx + ny)
@@ -111,7 +112,7 @@ def test_tiny_periodic_lattices_against_oracles(nx, ny):
         st = O.PhysState(O.phys_params(nx, ny, 0.7, force=force), solid, f0)
         st.run(steps)
         r0, u0, v0 = st.macro()
-        for variant in (None, 18):
+        for variant in (None, 18, 34):
             lat = L.Lattice(nx, ny, 0.7, force=force)
             lat.write_materials(W.materials_from_solid(solid))
             lat.write_populations(f0.astype(np.float32), 0)
@@ -372,6 +373,7 @@ def test_tile_variants_and_graphs_bit_identical(storage, nx, ny, x_edge):
     if nx % 1024 == 0:
         assert any(n[2][3] & 8 for n in names), "no lane-shuffle variant ran"
     assert sum(1 for n in names if n[2][3] & 16) >= 2 * 8, "two-step variants did not run"
+    assert sum(1 for n in names if n[2][3] & 128) >= 2 * 4, "row-stream variants did not run"
     assert np.isfinite(outs[0]).all()
     vis = visible_mask(solid, periodic_x=not x_edge)
     for k, o in enumerate(outs[1:], 1):
