@@ -90,7 +90,7 @@ struct lbm_handle {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     const lbm_mlp* mlp = nullptr;   // learned collision correction, if set
     uint64_t graph_mlp_gen = 0;     // the network's generation the graphs were captured with
-    int mlp_precision = LBM_MLP_BF16;
+    int mlp_precision = LBM_MLP_F16;
     uint32_t* health = nullptr;     // [max |u|^2 bits, non-finite flag]
     void* backup = nullptr;         // chunk start state of lbm_run_guarded
 
@@ -155,7 +155,11 @@ hipError_t enqueue_step(lbm_handle* h, int swap, bool export_macro)
         float* macro = export_macro ? h->macro : nullptr;
         if (h->mlp_precision == LBM_MLP_BF16)
             return launch_phys_step_learned_mfma(h->f16, h->d.x_edge, h->force(), h->phys(), src, dst, h->bits,
-                                                 h->words, lbm_mlp_device_packed(h->mlp),
+                                                 h->words, lbm_mlp_device_packed(h->mlp), false,
+                                                 lbm_mlp_device_params(h->mlp), s_in, s_out, macro, h->stream);
+        if (h->mlp_precision == LBM_MLP_F16)
+            return launch_phys_step_learned_mfma(h->f16, h->d.x_edge, h->force(), h->phys(), src, dst, h->bits,
+                                                 h->words, lbm_mlp_device_packed16(h->mlp), true,
                                                  lbm_mlp_device_params(h->mlp), s_in, s_out, macro, h->stream);
         return launch_phys_step_learned(h->f16, h->d.x_edge, h->force(), h->phys(), src, dst, h->bits, h->words,
                                         lbm_mlp_device_params(h->mlp), s_in, s_out, macro, h->stream);
@@ -892,7 +896,7 @@ int lbm_set_learned_precision(lbm_handle* h, int precision)
 {
     int rc = check_handle(h);
     if (rc) return rc;
-    if (precision != LBM_MLP_BF16 && precision != LBM_MLP_F32)
+    if (precision != LBM_MLP_BF16 && precision != LBM_MLP_F32 && precision != LBM_MLP_F16)
         return lbm_fail(LBM_ERR_INVALID, "unknown network precision %d", precision);
     h->mlp_precision = precision;
     drop_graphs(h);

@@ -449,11 +449,13 @@ __device__ inline void mlp_forward_one(const float* __restrict__ P, const float 
 }
 
 // ---------------------------------------------------------------------------
-// The network on matrix cores: bf16 operands, binary32 accumulation.
+// The network on matrix cores: bf16 or binary16 operands, binary32
+// accumulation.
 //
 // One wavefront evaluates 64 samples as two tiles of 32 with
-// v_mfma_f32_32x32x16_bf16 on the transposed problem H = W X (M = units,
-// N = samples, K = inputs).  Operand layouts of that instruction, lane l,
+// v_mfma_f32_32x32x16_bf16 (or _f16, the same layouts) on the transposed
+// problem H = W X (M = units, N = samples, K = inputs).  Operand layouts of
+// that instruction, lane l,
 // h = l / 32, element j:
 //   A (M x K)   row l % 32, k = 8 h + j            (j < 8)
 //   B (K x N)   column l % 32, k = 8 h + j          (j < 8)
@@ -477,15 +479,15 @@ __host__ __device__ constexpr int chain_unit(int k)
 // Row of C value j in lane half h of an M tile.
 __host__ __device__ constexpr int c_row(int j, int h) { return 8 * (j >> 2) + 4 * h + (j & 3); }
 
-// Lane-ordered bf16 weights (8 per lane per operand, 16 B, read by one
-// ds_read_b128):
+// Lane-ordered weights in the operand type (8 per lane per operand, 16 B,
+// read by one ds_read_b128):
 //   W1p[mt][lane][8]      mt < 2, from W1 (K = inputs padded to 16)
 //   W2p[mt][s][lane][8]   mt < 2, s < 4, from W2 with K permuted by chain_unit
 //   W3p[s][lane][8]       s < 4, rows = outputs padded to 32
 constexpr int kPackW1 = 0;
 constexpr int kPackW2 = kPackW1 + 2 * 64 * 8;
 constexpr int kPackW3 = kPackW2 + 2 * 4 * 64 * 8;
-constexpr int kPackElems = kPackW3 + 4 * 64 * 8;  // 7168 bf16 = 14 KiB
+constexpr int kPackElems = kPackW3 + 4 * 64 * 8;  // 7168 elements = 14 KiB
 
 // Packed weight element e (0 <= e < kPackElems) from the binary32 parameter
 // vector P.
@@ -553,7 +555,71 @@ __device__ __forceinline__ bf16x8 gelu8_bf16(const float* v)
     return out;
 }
 
-__device__ __forceinline__ bf16x8 lds_b128(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+// The same GELU for binary16 matrix operands: Phi as above, and the product
+// x Phi(x) taken in packed binary16 too (v_pk_mul_f16), so a pair of
+// activations costs eight vector instructions instead of ten (no binary32
+// products, no bf16 packing).  x is the binary16 value rounded toward zero
+// (11 significant bits against the 8 of a bf16 activation); beyond the
+// binary16 range it saturates at 65504 instead of overflowing.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f16x8 gelu8_f16(const float* v)
+{
+    f16x2 x[4], x2[4], c[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) x[p] = __builtin_bit_cast(f16x2, __builtin_amdgcn_cvt_pkrtz(v[2 * p], v[2 * p + 1]));
+#pragma unroll
+    for (int p = 0; p < 4; ++p) x2[p] = x[p] * x[p];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) c[p] = __builtin_elementwise_fma(x2[p], f16_splat(1.02411032e-05f), f16_splat(-4.08110474e-04f));
+#pragma unroll
+    for (int p = 0; p < 4; ++p) c[p] = __builtin_elementwise_fma(c[p], x2[p], f16_splat(6.66348797e-03f));
+#pragma unroll
+    for (int p = 0; p < 4; ++p) c[p] = __builtin_elementwise_fma(c[p], x2[p], f16_splat(-5.97102903e-02f));
+#pragma unroll
+    for (int p = 0; p < 4; ++p) c[p] = __builtin_elementwise_fma(c[p], x2[p], f16_splat(3.94299256e-01f));
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        c[p] = __builtin_elementwise_fma(c[p], x[p], f16_splat(0.5f));
+        c[p] = __builtin_elementwise_min(__builtin_elementwise_max(c[p], f16_splat(0.0f)), f16_splat(1.0f));
+    }
+    f16x8 out;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const f16x2 g = x[p] * c[p];
+        out[2 * p] = g.x;
+        out[2 * p + 1] = g.y;
+    }
+    return out;
+}
+
+// Operand element type E of the matrix-core network (bf16 or binary16):
+// its 8-wide operand, matrix instruction and GELU.
+template <typename E>
+struct Mma;
+template <>
+struct Mma<__bf16> {
+    typedef bf16x8 v8;
+    __device__ static __forceinline__ f32x16 mfma(v8 a, v8 b, f32x16 c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ v8 gelu8(const float* v) { return gelu8_bf16(v); }
+};
+template <>
+struct Mma<_Float16> {
+    typedef f16x8 v8;
+    __device__ static __forceinline__ f32x16 mfma(v8 a, v8 b, f32x16 c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ v8 gelu8(const float* v) { return gelu8_f16(v); }
+};
+
+template <typename E>
+__device__ __forceinline__ typename Mma<E>::v8 lds_b128(const E* p)
+{
+    return *reinterpret_cast<const typename Mma<E>::v8*>(p);
+}
 
 // LDS bias block of the matrix-core network: b1[64], b2[64], then b3 padded
 // with zeros to the 32 rows of its M tile (16-byte aligned).
@@ -581,12 +647,15 @@ __device__ __forceinline__ f32x16 bias_tile(const float* __restrict__ b, int h)
 }
 
 // Network on the 64 samples of one wavefront.  W: packed weights (LDS);
-// bias: b1[64], b2[64], b3[9] in binary32 (LDS); xs: 64 x 16 bf16 inputs
-// (row per sample, inputs 9..15 zero); ys: 64 x 9 binary32 outputs.  Every
-// lane of the wavefront must call this (matrix instructions use all lanes).
-__device__ __forceinline__ void mlp_wave_bf16(const __bf16* __restrict__ W, const float* __restrict__ bias,
-                                              const __bf16* __restrict__ xs, float* __restrict__ ys, int lane)
+// bias: b1[64], b2[64], b3[9] in binary32 (LDS); xs: 64 x 16 inputs (row per
+// sample, inputs 9..15 zero); ys: 64 x 9 binary32 outputs; operands of
+// element type E (bf16 or binary16), binary32 accumulation.  Every lane of
+// the wavefront must call this (matrix instructions use all lanes).
+template <typename E>
+__device__ __forceinline__ void mlp_wave(const E* __restrict__ W, const float* __restrict__ bias,
+                                         const E* __restrict__ xs, float* __restrict__ ys, int lane)
 {
+    typedef typename Mma<E>::v8 v8;
     const int h = lane >> 5, n = lane & 31;
     const float* b1 = bias;
     const float* b2 = bias + kMlpH;
@@ -596,42 +665,40 @@ __device__ __forceinline__ void mlp_wave_bf16(const __bf16* __restrict__ W, cons
     // latency (2 % faster than interleaving them in one wavefront).
 #pragma unroll 1
     for (int nt = 0; nt < 2; ++nt) {
-        const bf16x8 xb = lds_b128(xs + (32 * nt + n) * 16 + 8 * h);
-        bf16x8 hb[4];
+        const v8 xb = lds_b128(xs + (32 * nt + n) * 16 + 8 * h);
+        v8 hb[4];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
             f32x16 c = bias_tile(b1 + 32 * mt, h);
-            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b128(W + kPackW1 + (mt * 64 + lane) * 8), xb, c, 0, 0, 0);
+            c = Mma<E>::mfma(lds_b128(W + kPackW1 + (mt * 64 + lane) * 8), xb, c);
 {
                 float v[16];
 #pragma unroll
                 for (int j = 0; j < 16; ++j) v[j] = c[j];
-                hb[2 * mt] = gelu8_bf16(v);
-                hb[2 * mt + 1] = gelu8_bf16(v + 8);
+                hb[2 * mt] = Mma<E>::gelu8(v);
+                hb[2 * mt + 1] = Mma<E>::gelu8(v + 8);
             }
         }
-        bf16x8 hb2[4];
+        v8 hb2[4];
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
             f32x16 c = bias_tile(b2 + 32 * mt, h);
 #pragma unroll
             for (int s = 0; s < 4; ++s)
-                c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b128(W + kPackW2 + ((mt * 4 + s) * 64 + lane) * 8),
-                                                            hb[s], c, 0, 0, 0);
+                c = Mma<E>::mfma(lds_b128(W + kPackW2 + ((mt * 4 + s) * 64 + lane) * 8), hb[s], c);
 {
                 float v[16];
 #pragma unroll
                 for (int j = 0; j < 16; ++j) v[j] = c[j];
-                hb2[2 * mt] = gelu8_bf16(v);
-                hb2[2 * mt + 1] = gelu8_bf16(v + 8);
+                hb2[2 * mt] = Mma<E>::gelu8(v);
+                hb2[2 * mt + 1] = Mma<E>::gelu8(v + 8);
             }
         }
         // b3 is stored padded with zeros to 32 rows
         f32x16 c = bias_tile(b3, h);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
-            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b128(W + kPackW3 + (s * 64 + lane) * 8), hb2[s], c, 0, 0,
-                                                        0);
+            c = Mma<E>::mfma(lds_b128(W + kPackW3 + (s * 64 + lane) * 8), hb2[s], c);
         // Outputs 0..8 are rows 0-3 and 8 (lane half 0) and 4-7 (half 1):
         // both halves store c[0..3] at outputs 4h..4h+3 (the same register
         // indices in every lane, so no per-lane register selection).

@@ -94,9 +94,10 @@ hipError_t launch_batch_step2(int variant, bool f16, const BatchDomainDev* doms,
 hipError_t launch_phys_step_learned(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, void* dst,
                                    const uint32_t* bits, uint32_t words, const float* params, float s_in,
                                    float s_out, float* macro, hipStream_t s);
+// packed: the network's lane-ordered weights, binary16 when op_f16, else bf16
 hipError_t launch_phys_step_learned_mfma(bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
-                                        void* dst, const uint32_t* bits, uint32_t words, const __bf16* packed,
-                                        const float* params, float s_in, float s_out, float* macro,
+                                        void* dst, const uint32_t* bits, uint32_t words, const void* packed,
+                                        bool op_f16, const float* params, float s_in, float s_out, float* macro,
                                         hipStream_t s);
 hipError_t launch_restrict4(bool f16, int xedge, bool force, const PhysArgs& a, const void* src, const uint8_t* mat,
                             float nscale, float gcx, float gcy, float* fc, uint8_t* solid_c, float* uxc, float* uyc,
@@ -111,6 +112,8 @@ struct lbm_mlp;
 const float* lbm_mlp_device_params(const lbm_mlp* m);
 // Device pointer to its lane-ordered bf16 weights (kPackElems).
 const __bf16* lbm_mlp_device_packed(const lbm_mlp* m);
+// The same weights in binary16.
+const _Float16* lbm_mlp_device_packed16(const lbm_mlp* m);
 void lbm_mlp_scales(const lbm_mlp* m, float* s_in, float* s_out);
 // Changes whenever the network's parameters or scales change.
 uint64_t lbm_mlp_generation(const lbm_mlp* m);

@@ -133,22 +133,24 @@ __global__ __launch_bounds__(256) void k_phys_step_learned(PhysArgs a, const T* 
 }
 
 // One step with the learned correction evaluated on the matrix cores
-// (bf16 operands, binary32 accumulation; mlp_wave_bf16).  Blocks of 4
+// (operands of type E -- bf16 or binary16 -- and binary32 accumulation;
+// mlp_wave).  Blocks of 4
 // wavefronts loop over row segments of 256 cells, one cell per lane, so the
 // 14 KiB of packed weights are staged in LDS once per block.  Every lane
 // takes part in the matrix instructions; lanes beyond the row or on solid
 // cells feed zeros and store nothing.
-template <typename T, int XEDGE, bool FORCE>
+template <typename T, int XEDGE, bool FORCE, typename E>
 __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, const T* __restrict__ src,
                                                                 T* __restrict__ dst, const uint32_t* __restrict__ bits,
-                                                                uint32_t words, const __bf16* __restrict__ packed,
+                                                                uint32_t words, const E* __restrict__ packed,
                                                                 const float* __restrict__ params, float s_in,
                                                                 float s_out, uint32_t nseg,
                                                                 float* __restrict__ macro)
 {
-    __shared__ __align__(16) __bf16 W[kPackElems];
+    typedef typename Mma<E>::v8 v8;
+    __shared__ __align__(16) E W[kPackElems];
     __shared__ __align__(16) float bias[kBiasElems];
-    __shared__ __align__(16) __bf16 xs[4][64 * 16];
+    __shared__ __align__(16) E xs[4][64 * 16];
     __shared__ float ys[4][64 * kMlpOut];
     {
         const uint4* from = reinterpret_cast<const uint4*>(packed);
@@ -199,27 +201,27 @@ __global__ __launch_bounds__(256) void k_phys_step_learned_mfma(PhysArgs a, cons
         // BGK now, the correction after the network: only g, the three
         // moments and the neighbour bits stay live across it.
         phys_collide<FORCE>(a, g, rho, ux, uy);
-        bf16x8 lo, hi;
+        v8 lo, hi;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            lo[i] = (__bf16)xin[i];
-            hi[i] = (__bf16)(i == 0 ? xin[8] : 0.0f);
+            lo[i] = (E)xin[i];
+            hi[i] = (E)(i == 0 ? xin[8] : 0.0f);
         }
         // cells that are not corrected feed zeros (their column of the
         // product is ignored anyway; zeros keep it finite)
         if (!learned) {
-            lo = bf16x8{};
-            hi = bf16x8{};
+            lo = v8{};
+            hi = v8{};
         }
-        *reinterpret_cast<bf16x8*>(&xs[wave][lane * 16]) = lo;
-        *reinterpret_cast<bf16x8*>(&xs[wave][lane * 16 + 8]) = hi;
+        *reinterpret_cast<v8*>(&xs[wave][lane * 16]) = lo;
+        *reinterpret_cast<v8*>(&xs[wave][lane * 16 + 8]) = hi;
         // xs[wave] and ys[wave] belong to this wavefront alone, whose LDS
         // accesses complete in issue order: a wavefront-scope fence keeps
         // the compiler from moving them across, and no block barrier is
         // needed (the wavefronts of a block run independently).
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        mlp_wave_bf16(W, bias, xs[wave], ys[wave], lane);
+        mlp_wave(W, bias, xs[wave], ys[wave], lane);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         if (learned) {
@@ -293,15 +295,22 @@ hipError_t launch_phys_step_learned(bool f16, int xedge, bool force, const PhysA
 }
 
 hipError_t launch_phys_step_learned_mfma(bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
-                                        void* dst, const uint32_t* bits, uint32_t words, const __bf16* packed,
-                                        const float* params, float s_in, float s_out, float* macro, hipStream_t s)
+                                        void* dst, const uint32_t* bits, uint32_t words, const void* packed,
+                                        bool op_f16, const float* params, float s_in, float s_out, float* macro,
+                                        hipStream_t s)
 {
     const uint32_t nseg = (uint32_t)((a.nx + 255) / 256);
     const uint32_t tiles = nseg * (uint32_t)a.ny;
     const dim3 grid(tiles < 2048u ? tiles : 2048u), block(256);
-#define LBM_L(T, XE, FO)                                                                                       \
-    hipLaunchKernelGGL((k_phys_step_learned_mfma<T, XE, FO>), grid, block, 0, s, a, (const T*)src, (T*)dst, bits, \
-                       words, packed, params, s_in, s_out, nseg, macro)
+#define LBM_L(T, XE, FO)                                                                                          \
+    do {                                                                                                          \
+        if (op_f16)                                                                                               \
+            hipLaunchKernelGGL((k_phys_step_learned_mfma<T, XE, FO, _Float16>), grid, block, 0, s, a, (const T*)src, \
+                               (T*)dst, bits, words, (const _Float16*)packed, params, s_in, s_out, nseg, macro);   \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_phys_step_learned_mfma<T, XE, FO, __bf16>), grid, block, 0, s, a, (const T*)src, \
+                               (T*)dst, bits, words, (const __bf16*)packed, params, s_in, s_out, nseg, macro);     \
+    } while (0)
     if (f16) {
         if (xedge) { if (force) LBM_L(_Float16, 1, true); else LBM_L(_Float16, 1, false); }
         else { if (force) LBM_L(_Float16, 0, true); else LBM_L(_Float16, 0, false); }

@@ -56,19 +56,24 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ p
 }
 
 // ---------------------------------------------------------------------------
-// Matrix-core forward (bf16 operands, binary32 accumulation): lane-ordered
-// bf16 weights are rebuilt from the binary32 parameters whenever those
-// change; a block of 4 wavefronts takes 256 samples.
+// Matrix-core forward (bf16 or binary16 operands, binary32 accumulation):
+// lane-ordered weights in both operand types are rebuilt from the binary32
+// parameters whenever those change; a block of 4 wavefronts takes 256
+// samples.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_mlp_pack_bf16(const float* __restrict__ params, __bf16* __restrict__ packed)
+__global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ params, __bf16* __restrict__ packed,
+                                                  _Float16* __restrict__ packed16)
 {
     const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e < kPackElems) packed[e] = (__bf16)pack_weight(params, e);
+    if (e >= kPackElems) return;
+    const float w = pack_weight(params, e);
+    packed[e] = (__bf16)w;
+    packed16[e] = (_Float16)w;
 }
 
 // Stages the packed weights and the biases b1, b2, b3 of a network in LDS.
-__device__ inline void stage_bf16_net(const __bf16* __restrict__ packed, const float* __restrict__ params,
-                                      __bf16* W, float* bias)
+template <typename E>
+__device__ inline void stage_net(const E* __restrict__ packed, const float* __restrict__ params, E* W, float* bias)
 {
     const uint4* src = reinterpret_cast<const uint4*>(packed);
     uint4* dst = reinterpret_cast<uint4*>(W);
@@ -81,22 +86,23 @@ __device__ inline void stage_bf16_net(const __bf16* __restrict__ packed, const f
         bias[2 * kMlpH + k] = k < kMlpOut ? params[kOffB3 + k] : 0.0f;
 }
 
-__global__ __launch_bounds__(256) void k_mlp_forward_bf16(const __bf16* __restrict__ packed,
-                                                          const float* __restrict__ params,
-                                                          const float* __restrict__ X, float* __restrict__ Y, int n)
+template <typename E>
+__global__ __launch_bounds__(256) void k_mlp_forward_mma(const E* __restrict__ packed,
+                                                         const float* __restrict__ params,
+                                                         const float* __restrict__ X, float* __restrict__ Y, int n)
 {
-    __shared__ __align__(16) __bf16 W[kPackElems];
+    __shared__ __align__(16) E W[kPackElems];
     __shared__ __align__(16) float bias[kBiasElems];
-    __shared__ __align__(16) __bf16 xs[4][64 * 16];
+    __shared__ __align__(16) E xs[4][64 * 16];
     __shared__ float ys[4][64 * kMlpOut];
-    stage_bf16_net(packed, params, W, bias);
+    stage_net(packed, params, W, bias);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int s = blockIdx.x * 256 + threadIdx.x;
 #pragma unroll
     for (int i = 0; i < 16; ++i)
-        xs[wave][lane * 16 + i] = (__bf16)((s < n && i < kMlpIn) ? X[(size_t)s * kMlpIn + i] : 0.0f);
+        xs[wave][lane * 16 + i] = (E)((s < n && i < kMlpIn) ? X[(size_t)s * kMlpIn + i] : 0.0f);
     __syncthreads();
-    mlp_wave_bf16(W, bias, xs[wave], ys[wave], lane);
+    mlp_wave(W, bias, xs[wave], ys[wave], lane);
     __syncthreads();
     if (s >= n) return;
 #pragma unroll
@@ -414,7 +420,8 @@ struct lbm_mlp {
     int dev = 0;
     hipStream_t stream = nullptr;
     float* params = nullptr;  // binary32 master copy
-    __bf16* packed = nullptr; // lane-ordered bf16 weights of the matrix-core path
+    __bf16* packed = nullptr;      // lane-ordered bf16 weights of the matrix-core path
+    _Float16* packed16 = nullptr;  // the same in binary16
     float* adam_m = nullptr;
     float* adam_v = nullptr;
     float* grad = nullptr;
@@ -454,7 +461,7 @@ int mlp_quiesce_device(lbm_mlp* m)
 
 void mlp_free(lbm_mlp* m)
 {
-    for (void* p : {(void*)m->params, (void*)m->packed, (void*)m->adam_m, (void*)m->adam_v, (void*)m->grad, (void*)m->partial,
+    for (void* p : {(void*)m->params, (void*)m->packed, (void*)m->packed16, (void*)m->adam_m, (void*)m->adam_v, (void*)m->grad, (void*)m->partial,
                     (void*)m->loss_part, (void*)m->loss, (void*)m->X, (void*)m->T})
         if (p) (void)hipFree(p);
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -484,11 +491,11 @@ int mlp_grad(lbm_mlp* m, const float* X, const float* T, int n, int batch, uint6
     return LBM_OK;
 }
 
-// Rebuilds the bf16 weights after the binary32 parameters changed.
+// Rebuilds the matrix-core weights after the binary32 parameters changed.
 int mlp_repack(lbm_mlp* m)
 {
-    hipLaunchKernelGGL(k_mlp_pack_bf16, dim3((kPackElems + 255) / 256), dim3(256), 0, m->stream, m->params,
-                       m->packed);
+    hipLaunchKernelGGL(k_mlp_pack, dim3((kPackElems + 255) / 256), dim3(256), 0, m->stream, m->params, m->packed,
+                       m->packed16);
     MLP_TRY(hipGetLastError());
     return LBM_OK;
 }
@@ -496,6 +503,7 @@ int mlp_repack(lbm_mlp* m)
 }  // namespace
 
 const __bf16* lbm_mlp_device_packed(const lbm_mlp* m) { return m->packed; }
+const _Float16* lbm_mlp_device_packed16(const lbm_mlp* m) { return m->packed16; }
 
 // Device pointer to the binary32 parameters and the two scales of a
 // network (used by the learned collision step).
@@ -522,6 +530,7 @@ int lbm_mlp_create(uint64_t seed, int capacity, lbm_mlp** out)
     const size_t pb = kMlpParams * sizeof(float);
     if (e == hipSuccess) e = hipMalloc(&m->params, pb);
     if (e == hipSuccess) e = hipMalloc(&m->packed, kPackElems * sizeof(__bf16));
+    if (e == hipSuccess) e = hipMalloc(&m->packed16, kPackElems * sizeof(_Float16));
     if (e == hipSuccess) e = hipMalloc(&m->adam_m, pb);
     if (e == hipSuccess) e = hipMalloc(&m->adam_v, pb);
     if (e == hipSuccess) e = hipMalloc(&m->grad, pb);
@@ -616,7 +625,7 @@ int lbm_mlp_get_scales(lbm_mlp* m, float* in_scale, float* out_scale)
     return LBM_OK;
 }
 
-static int mlp_forward_impl(lbm_mlp* m, const float* x, int n, float* y, bool bf16)
+static int mlp_forward_impl(lbm_mlp* m, const float* x, int n, float* y, int precision)
 {
     int rc = mlp_check(m);
     if (rc) return rc;
@@ -627,9 +636,12 @@ static int mlp_forward_impl(lbm_mlp* m, const float* x, int n, float* y, bool bf
     hipError_t e = hipMalloc(&dy, (size_t)n * kMlpOut * sizeof(float));
     if (e == hipSuccess) e = hipMemcpyAsync(dx, x, (size_t)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, m->stream);
     if (e == hipSuccess) {
-        if (bf16)
-            hipLaunchKernelGGL(k_mlp_forward_bf16, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->packed,
-                               m->params, dx, dy, n);
+        if (precision == LBM_MLP_BF16)
+            hipLaunchKernelGGL(k_mlp_forward_mma<__bf16>, dim3((n + 255) / 256), dim3(256), 0, m->stream,
+                               m->packed, m->params, dx, dy, n);
+        else if (precision == LBM_MLP_F16)
+            hipLaunchKernelGGL(k_mlp_forward_mma<_Float16>, dim3((n + 255) / 256), dim3(256), 0, m->stream,
+                               m->packed16, m->params, dx, dy, n);
         else
             hipLaunchKernelGGL(k_mlp_forward, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->params, dx, dy, n);
         e = hipGetLastError();
@@ -642,9 +654,20 @@ static int mlp_forward_impl(lbm_mlp* m, const float* x, int n, float* y, bool bf
     return LBM_OK;
 }
 
-int lbm_mlp_forward(lbm_mlp* m, const float* x, int n, float* y) { return mlp_forward_impl(m, x, n, y, false); }
+int lbm_mlp_forward(lbm_mlp* m, const float* x, int n, float* y)
+{
+    return mlp_forward_impl(m, x, n, y, LBM_MLP_F32);
+}
 
-int lbm_mlp_forward_bf16(lbm_mlp* m, const float* x, int n, float* y) { return mlp_forward_impl(m, x, n, y, true); }
+int lbm_mlp_forward_bf16(lbm_mlp* m, const float* x, int n, float* y)
+{
+    return mlp_forward_impl(m, x, n, y, LBM_MLP_BF16);
+}
+
+int lbm_mlp_forward_f16(lbm_mlp* m, const float* x, int n, float* y)
+{
+    return mlp_forward_impl(m, x, n, y, LBM_MLP_F16);
+}
 
 int lbm_mlp_loss_grad(lbm_mlp* m, const float* x, const float* t, int n, float* grad, double* loss)
 {

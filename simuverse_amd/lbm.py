@@ -29,6 +29,7 @@ LAYOUT_ROWS = 0
 LAYOUT_PLANES = 1
 MLP_BF16 = 0
 MLP_F32 = 1
+MLP_F16 = 2
 MAT_BULK, MAT_BOUNDARY, MAT_INLET, MAT_OBSTACLE, MAT_OUTLET, MAT_EXTERNAL_FORCE, MAT_GHOST = 1, 2, 3, 4, 5, 6, 7
 
 LATTICE_INFO_DTYPE = np.dtype([("material", "<i4"), ("block_iter", "<i4"), ("vx", "<f4"), ("vy", "<f4")])
@@ -124,6 +125,7 @@ _SIGNATURES = [
     ("lbm_mlp_get_scales", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("lbm_mlp_forward", C.c_int, [_P, _P, C.c_int, _P]),
     ("lbm_mlp_forward_bf16", C.c_int, [_P, _P, C.c_int, _P]),
+    ("lbm_mlp_forward_f16", C.c_int, [_P, _P, C.c_int, _P]),
     ("lbm_mlp_loss_grad", C.c_int, [_P, _P, _P, C.c_int, _P, C.POINTER(C.c_double)]),
     ("lbm_mlp_loss_grad_noalloc", C.c_int, [_P, _P, _P, C.c_int, _P, C.POINTER(C.c_double)]),
     ("lbm_mlp_clear_samples", C.c_int, [_P]),
@@ -395,7 +397,8 @@ class Lattice:
         self._net = net  # keep the network alive while attached
 
     def set_learned_precision(self, precision: int) -> None:
-        """MLP_BF16 (matrix cores, the default) or MLP_F32 (vector ALUs)."""
+        """MLP_F16 (matrix cores, binary16 operands and GELU; the default),
+        MLP_BF16 (matrix cores, bf16 operands) or MLP_F32 (vector ALUs)."""
         _check(self._lib.lbm_set_learned_precision(self._h, precision))
 
     def health(self) -> tuple[float, bool]:
@@ -578,10 +581,13 @@ class Network:
         _check(self._lib.lbm_mlp_get_scales(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
-    def forward(self, x: np.ndarray, bf16: bool = False) -> np.ndarray:
+    def forward(self, x: np.ndarray, bf16: bool = False, f16: bool = False) -> np.ndarray:
+        """Outputs of n samples: binary32 on the vector ALUs, or on the
+        matrix cores with bf16 (bf16=True) or binary16 (f16=True) operands."""
         x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 9)
         y = np.empty_like(x)
-        fn = self._lib.lbm_mlp_forward_bf16 if bf16 else self._lib.lbm_mlp_forward
+        fn = (self._lib.lbm_mlp_forward_bf16 if bf16 else
+              self._lib.lbm_mlp_forward_f16 if f16 else self._lib.lbm_mlp_forward)
         _check(fn(self._h, _ptr(x), x.shape[0], _ptr(y)))
         return y
 

@@ -48,7 +48,7 @@ def test_c5_workflow_1024_to_256():
     assert r.plain_err < 0.1 and r.learned_err < 0.1
 
 
-@pytest.mark.parametrize("precision", [L.MLP_BF16, L.MLP_F32], ids=["bf16", "f32"])
+@pytest.mark.parametrize("precision", [L.MLP_BF16, L.MLP_F16, L.MLP_F32], ids=["bf16", "mma16", "f32"])
 def test_learned_step_full_coarse_size_properties(precision):
     """At U = 0.05 and g = 4e-7 the per-step momentum identity holds only to
     the binary32 rounding of the stored deviations, which is coherent over

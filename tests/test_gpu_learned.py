@@ -62,41 +62,46 @@ def test_gradient_matches_fp64_oracle():
         assert np.linalg.norm(g[a:b] - gref[a:b]) <= 1e-4 * np.linalg.norm(gref[a:b]) + 1e-12
 
 
-def test_forward_bf16_matrix_cores_vs_fp64_oracle():
+@pytest.mark.parametrize("op", ["bf16", "f16"])
+def test_forward_matrix_cores_vs_fp64_oracle(op):
     """Matrix-core forward: inputs, weights and hidden activations rounded
-    to bf16 (8 significant bits, relative rounding <= 2^-9), binary32
-    accumulation.  A few such roundings per layer give an output error of
-    a few 1e-3 of its scale: bounded here by 1e-2 (rms) and 3e-2 (max).  A
-    wrong operand layout or unit permutation gives O(1) errors."""
-    print("case: network forward bf16 (matrix cores) vs fp64", flush=True)
+    to bf16 (8 significant bits, relative rounding <= 2^-9) or to binary16
+    (11 bits, the activations toward zero: <= 2^-10), binary32
+    accumulation, and the GELU polynomial (|error| <= 5.2e-3).  A few such
+    roundings per layer give an output error of a few 1e-3 of its scale:
+    bounded here by 1e-2 (rms) and 3e-2 (max).  A wrong operand layout or
+    unit permutation gives O(1) errors."""
+    print(f"case: network forward {op} (matrix cores) vs fp64", flush=True)
     net = L.Network(seed=3)
     p = random_params(1, 0.15)
     net.set_params(p)
     x = np.random.default_rng(2).standard_normal((1000, 9)).astype(np.float32)
-    y = net.forward(x, bf16=True)
+    y = net.forward(x, bf16=op == "bf16", f16=op == "f16")
     ref = O.mlp_forward(p.astype(np.float64), x.astype(np.float64))
     err = y - ref
     rms = float(np.sqrt(np.mean(ref**2)))
-    print(f"bf16 forward: rms err {np.sqrt(np.mean(err**2)) / rms:.3e}, max err {np.max(np.abs(err)) / np.max(np.abs(ref)):.3e}",
+    print(f"{op} forward: rms err {np.sqrt(np.mean(err**2)) / rms:.3e}, max err {np.max(np.abs(err)) / np.max(np.abs(ref)):.3e}",
           flush=True)
     assert np.sqrt(np.mean(err**2)) < 1e-2 * rms
     assert np.max(np.abs(err)) < 3e-2 * np.max(np.abs(ref))
     # every output column carries signal and matches on its own
     for m in range(9):
         assert np.linalg.norm(err[:, m]) < 2e-2 * np.linalg.norm(ref[:, m])
-    # bf16 parameters follow set_params
+    # the packed parameters follow set_params
     net.set_params(np.zeros(5385, np.float32))
-    assert np.array_equal(net.forward(x, bf16=True), np.zeros_like(x))
+    assert np.array_equal(net.forward(x, bf16=op == "bf16", f16=op == "f16"), np.zeros_like(x))
 
 
-def test_forward_bf16_large_activations_vs_fp64_oracle():
+@pytest.mark.parametrize("op", ["bf16", "f16"])
+def test_forward_large_activations_vs_fp64_oracle(op):
     """Matrix-core forward with hidden pre-activations spread over
     |x| ~ 1e-1 .. 9e3 (median 50 in layer 1, 770 in layer 2, 81 % of layer 2
     beyond 256): the GELU saturates (exactly x or 0 for |x| >= 3.5) and the
     binary16 evaluation of its polynomial overflows (x^2 and the polynomial
     become +inf for |x| >= 256), which must still give x or 0.  Same bounds
-    as the unscaled forward."""
-    print("case: network forward bf16, large activations, vs fp64", flush=True)
+    as the unscaled forward (binary16 operands: the activations stay below
+    the binary16 limit of 65504)."""
+    print(f"case: network forward {op}, large activations, vs fp64", flush=True)
     net = L.Network(seed=3)
     p = random_params(1, 0.15).astype(np.float64)
     p[:640] *= 200.0  # W1 and b1
@@ -105,11 +110,11 @@ def test_forward_bf16_large_activations_vs_fp64_oracle():
     net.set_params(p)
     x = np.random.default_rng(2).standard_normal((1000, 9)).astype(np.float32)
     x[::7] *= 1e-3  # and some near zero
-    y = net.forward(x, bf16=True)
+    y = net.forward(x, bf16=op == "bf16", f16=op == "f16")
     ref = O.mlp_forward(p.astype(np.float64), x.astype(np.float64))
     err = y - ref
     rms = float(np.sqrt(np.mean(ref**2)))
-    print(f"bf16 forward (large): rms err {np.sqrt(np.mean(err**2)) / rms:.3e}, "
+    print(f"{op} forward (large): rms err {np.sqrt(np.mean(err**2)) / rms:.3e}, "
           f"max err {np.max(np.abs(err)) / np.max(np.abs(ref)):.3e}", flush=True)
     assert np.all(np.isfinite(y))
     assert np.sqrt(np.mean(err**2)) < 1e-2 * rms
@@ -131,7 +136,7 @@ def test_training_reduces_loss():
     assert l1 < 0.1 * l0
 
 
-@pytest.mark.parametrize("precision", [L.MLP_BF16, L.MLP_F32], ids=["bf16", "f32"])
+@pytest.mark.parametrize("precision", [L.MLP_BF16, L.MLP_F16, L.MLP_F32], ids=["bf16", "mma16", "f32"])
 def test_correction_conserves_mass_and_momentum(precision):
     print(f"case: learned correction conserves ({precision})", flush=True)
     nx, ny = 96, 64
@@ -155,7 +160,7 @@ def test_correction_conserves_mass_and_momentum(precision):
     assert not np.array_equal(lat.read_populations(), plain.read_populations())
 
 
-@pytest.mark.parametrize("precision", [L.MLP_BF16, L.MLP_F32], ids=["bf16", "f32"])
+@pytest.mark.parametrize("precision", [L.MLP_BF16, L.MLP_F16, L.MLP_F32], ids=["bf16", "mma16", "f32"])
 @pytest.mark.parametrize("x_edge", [L.X_PERIODIC, L.X_INLET_OUTLET], ids=["periodic", "channel"])
 def test_zero_network_step_equals_plain_step_f16(precision, x_edge):
     """A network with all parameters zero adds nothing (y = 0, so every
@@ -191,12 +196,13 @@ def test_zero_network_step_equals_plain_step_f16(precision, x_edge):
     assert bad == 0, f"{bad} visible slots differ"
 
 
-def test_learned_step_bf16_tracks_f32():
-    """The matrix-core learned step against the binary32 one on a lattice
-    whose width (160) leaves part of a 256-cell segment empty, with disks
-    and a body force: after 50 steps the two differ by a few percent of what
-    the correction itself changed."""
-    print("case: learned step bf16 vs f32", flush=True)
+def test_learned_step_matrix_cores_track_f32():
+    """The matrix-core learned steps (bf16 and binary16 operands) against
+    the binary32 one on a lattice whose width (160) leaves part of a
+    256-cell segment empty, with disks and a body force: after 50 steps
+    each differs by a few percent of what the correction itself changed;
+    a lattice that never chose a precision runs the binary16 one."""
+    print("case: learned step matrix cores vs f32", flush=True)
     nx, ny = 160, 96
     scene = W.periodic_disks(nx, ny, 0.05, 4, 8, seed=5)
     f0 = W.rest_state(nx, ny, scene.solid, amplitude=1e-3, seed=4).astype(np.float32)
@@ -204,20 +210,25 @@ def test_learned_step_bf16_tracks_f32():
     net.set_params(random_params(10, 0.2))
     net.set_scales(1e-3, 0.05)
     out = {}
-    for name, prec in (("plain", None), ("f32", L.MLP_F32), ("bf16", L.MLP_BF16)):
+    for name, prec in (("plain", None), ("f32", L.MLP_F32), ("bf16", L.MLP_BF16), ("f16", L.MLP_F16),
+                       ("default", -1)):
         lat = L.Lattice(nx, ny, 0.6, force=(1e-5, 0.0))
         lat.write_materials(W.materials_from_solid(scene.solid))
         lat.write_populations(f0, 0)
         if prec is not None:
             lat.set_learned_collision(net)
-            lat.set_learned_precision(prec)
+            if prec >= 0:
+                lat.set_learned_precision(prec)
         lat.run(50)
         out[name] = lat.read_populations()[:, ~scene.solid].astype(np.float64)
     d_corr = np.linalg.norm(out["f32"] - out["plain"])
     d_bf16 = np.linalg.norm(out["bf16"] - out["f32"])
-    print(f"correction effect {d_corr:.3e}, bf16 vs f32 {d_bf16:.3e}", flush=True)
+    d_f16 = np.linalg.norm(out["f16"] - out["f32"])
+    print(f"correction effect {d_corr:.3e}, bf16 vs f32 {d_bf16:.3e}, f16 vs f32 {d_f16:.3e}", flush=True)
     assert d_corr > 0
     assert d_bf16 < 5e-2 * d_corr
+    assert d_f16 < 5e-2 * d_corr
+    assert np.array_equal(out["default"], out["f16"])
 
 
 def test_blow_up_falls_back_to_bgk():
@@ -424,14 +435,14 @@ def device_learned_run(solid, f0, tau, kw, net, precision, steps, storage=L.STOR
 @pytest.mark.parametrize("kind", ["channel", "periodic"])
 def test_learned_step_matches_fp64_oracle(kind):
     """k_phys_step_learned (binary32 network) and k_phys_step_learned_mfma
-    (bf16 operands, binary32 accumulation) against the binary64 oracle of
-    the learned step, on lattices with walls, after 1 and 20 steps from a
-    perturbed state.  Bounds: binary32, |f - f_oracle| <= 4e-7 on every
-    fluid population (13 ulp of f_0 ~ 0.44 after 20 steps, measured up to
-    1.7e-7; the correction itself changes f by ~1e-5);
-    bf16, the difference from the oracle <= 5 % of what the correction
-    changes (rms over fluid populations), the rounding of the network's
-    operands to 8 significant bits."""
+    (bf16 or binary16 operands, binary32 accumulation) against the binary64
+    oracle of the learned step, on lattices with walls, after 1 and 20 steps
+    from a perturbed state.  Bounds: binary32, |f - f_oracle| <= 4e-7 on
+    every fluid population (13 ulp of f_0 ~ 0.44 after 20 steps, measured up
+    to 1.7e-7; the correction itself changes f by ~1e-5); matrix cores, the
+    difference from the oracle <= 5 % of what the correction changes (rms
+    over fluid populations), the rounding of the network's operands to 8
+    (bf16) or 11 (binary16) significant bits and the GELU polynomial."""
     nx, ny = 96, 64
     tau = 0.56
     solid, kw = walled_scene(kind, nx, ny)
@@ -455,10 +466,13 @@ def test_learned_step_matches_fp64_oracle(kind):
         err32 = np.max(np.abs(f32[:, fl] - ref[:, fl]))
         bf = device_learned_run(solid, f0, tau, kw, net, L.MLP_BF16, steps)
         errbf = np.sqrt(np.mean((bf[:, fl] - ref[:, fl]) ** 2)) / np.sqrt(np.mean(effect**2))
+        h16 = device_learned_run(solid, f0, tau, kw, net, L.MLP_F16, steps)
+        err16 = np.sqrt(np.mean((h16[:, fl] - ref[:, fl]) ** 2)) / np.sqrt(np.mean(effect**2))
         print(f"f32 max err {err32:.3e}; bf16 rms err / correction {errbf:.3e}; "
-              f"correction rms {np.sqrt(np.mean(effect**2)):.3e}", flush=True)
+              f"f16 rms err / correction {err16:.3e}; correction rms {np.sqrt(np.mean(effect**2)):.3e}", flush=True)
         assert err32 <= 4e-7
         assert errbf <= 5e-2
+        assert err16 <= 5e-2
 
 
 @pytest.mark.parametrize("tau", [0.505, 2.0])
@@ -477,7 +491,7 @@ def test_learned_step_saturated_modulation_matches_oracle(tau):
     net.set_scales(1.0, 1.0)
     po = O.phys_params(nx, ny, tau, force=kw["force"])
     fl = ~solid
-    for prec in (L.MLP_F32, L.MLP_BF16):
+    for prec in (L.MLP_F32, L.MLP_BF16, L.MLP_F16):
         print(f"case: saturated modulation, tau {tau}, precision {prec}", flush=True)
         ref = O.learned_run(po, solid, f0, p.astype(np.float64), 1.0, 1.0, 10)
         dev = device_learned_run(solid, f0, tau, kw, net, prec, 10)
@@ -517,7 +531,7 @@ def test_restriction_matches_fp64_oracle():
 
 
 @pytest.mark.parametrize("storage", [L.STORAGE_F32, L.STORAGE_F16_DEV], ids=["f32", "f16"])
-@pytest.mark.parametrize("precision", [L.MLP_BF16, L.MLP_F32], ids=["bf16", "f32"])
+@pytest.mark.parametrize("precision", [L.MLP_BF16, L.MLP_F16, L.MLP_F32], ids=["bf16", "mma16", "f32"])
 def test_learned_handle_exports_macro(storage, precision):
     """FluidSimulator::compute on a learned lattice: lbm_run(h, 2, 1) exports
     the macro field of the last step's gathered populations, which equals
