@@ -152,11 +152,14 @@ def cpu_baseline(scene, budget_s: float = 12.0):
 
 def describe_variant(shape):
     """The step-kernel variant as a dict: one-step shapes (block,
-    cells_per_thread, vec), the two-step LDS tile (rows x width) or the
-    two-step row stream (strips of block columns, chunks of rows)."""
+    cells_per_thread, vec), the two- or three-step LDS tile (rows x width)
+    or the two-step row stream (strips of block columns, chunks of rows)."""
     b, c, v, f = shape
     flags = [name for bit, name in ((1, "nt_store"), (2, "nt_load"), (4, "xcd_rows"), (8, "lane_shuffle"),
-                                    (16, "two_step"), (128, "row_stream")) if f & bit]
+                                    (16, "two_step"), (64, "three_step"), (128, "row_stream")) if f & bit]
+    if f & 64:
+        return {"kernel": "k_phys_step3", "block": b, "tile_rows": c, "tile_width": v, "flags": flags,
+                "steps_per_launch": 3}
     if f & 128:
         return {"kernel": "k_phys_step2_rows", "block": b, "chunk_rows": c, "strip_width": v, "flags": flags,
                 "steps_per_launch": 2}
@@ -207,7 +210,7 @@ def run_batch(args):
     bpc = 36 if args.f16 else 72
     variant = b.variant()
     block, vec, tile_h, flags = L.batch_variant_shapes()[variant]
-    per_launch = 2 if flags & 16 else 1
+    per_launch = 3 if flags & 64 else (2 if flags & 16 else 1)
     workload = f"d2q9_batch512_{'f16dev' if args.f16 else 'f32'}"
     roof = roofline(bpc, cells, step_s, per_launch, traffic_from_profiles(workload, variant))
     # Domains whose state is no longer finite (BGK at tau near 1/2 with a

@@ -21,8 +21,8 @@ LIB = LIBDIR / "libsimuverse_lbm.so"
 HOST_LIB = LIBDIR / "libsimuverse_host.so"
 HOST_SOURCES = ["host/d2q9_node.cpp"]
 
-SOURCES = ["lbm_kernels.hip", "lbm_step2.hip", "lbm_learned.hip", "lbm_api.hip", "lbm_batch_api.hip", "lbm_mlp.hip",
-           "lbm3d.hip"]
+SOURCES = ["lbm_kernels.hip", "lbm_step2.hip", "lbm_step3.hip", "lbm_learned.hip", "lbm_api.hip", "lbm_batch_api.hip",
+           "lbm_mlp.hip", "lbm3d.hip"]
 
 # -ffp-contract=off: the parity kernel must round exactly where the
 # reference's WGSL rounds, and the physics kernels stay reproducible

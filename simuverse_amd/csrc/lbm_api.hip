@@ -165,7 +165,7 @@ hipError_t enqueue_step(lbm_handle* h, int swap, bool export_macro)
                                         lbm_mlp_device_params(h->mlp), s_in, s_out, macro, h->stream);
     }
     if (h->physics()) {
-        const int v = step_variant_two_step(h->variant) ? kSingleStepFallback : h->variant;
+        const int v = step_variant_steps(h->variant) > 1 ? kSingleStepFallback : h->variant;
         return launch_phys_step(v, h->f16, h->d.x_edge, h->force(), export_macro, h->phys(), src, dst, h->bits,
                                 h->words, h->macro, h->stream);
     }
@@ -173,19 +173,27 @@ hipError_t enqueue_step(lbm_handle* h, int swap, bool export_macro)
                             h->macro, h->stream);
 }
 
-// Whether steps go two per launch (a two-step variant on a plain physics
-// handle).
-bool two_step(const lbm_handle* h) { return h->physics() && !h->mlp && step_variant_two_step(h->variant); }
+// Steps one launch advances: 2 or 3 for a multi-step variant on a plain
+// physics handle, else 1.
+int launch_steps(const lbm_handle* h) { return h->physics() && !h->mlp ? step_variant_steps(h->variant) : 1; }
 
-// Two steps from buffer `swap` into buffer 1-swap in one launch.
-hipError_t enqueue_step2(lbm_handle* h, int swap)
+// launch_steps(h) steps from buffer `swap` into buffer 1-swap in one launch.
+hipError_t enqueue_multi(lbm_handle* h, int swap)
 {
+    if (launch_steps(h) == 3)
+        return launch_phys_step3(h->variant, h->f16, h->d.x_edge, h->force(), h->phys(), h->buf[swap],
+                                 h->buf[swap ^ 1], h->bits, h->words, h->stream);
     return launch_phys_step2(h->variant, h->f16, h->d.x_edge, h->force(), h->phys(), h->buf[swap],
                              h->buf[swap ^ 1], h->bits, h->words, h->stream);
 }
 
-// Buffer flips of one graph chunk: one per launch.
-int chunk_flips(const lbm_handle* h) { return two_step(h) ? (h->graph_chunk / 2) & 1 : 0; }
+// Buffer flips of one graph chunk: one per launch (its multi-step launches,
+// then one-step launches for the steps left over).
+int chunk_flips(const lbm_handle* h)
+{
+    const int spl = launch_steps(h);
+    return (h->graph_chunk / spl + h->graph_chunk % spl) & 1;
+}
 
 int ensure_graph(lbm_handle* h, int b)
 {
@@ -197,13 +205,14 @@ int ensure_graph(lbm_handle* h, int b)
     if (h->graph[b]) return LBM_OK;
     HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     hipError_t launch_err = hipSuccess;
-    if (two_step(h)) {
-        for (int s = 0; s < h->graph_chunk / 2 && launch_err == hipSuccess; ++s)
-            launch_err = enqueue_step2(h, (b + s) & 1);
-    } else {
-        for (int s = 0; s < h->graph_chunk && launch_err == hipSuccess; ++s)
-            launch_err = enqueue_step(h, (b + s) & 1, false);
-    }
+    // multi-step launches, then one-step launches for the steps left over
+    const int spl = launch_steps(h);
+    const int multi = spl > 1 ? h->graph_chunk / spl : 0, singles = h->graph_chunk - multi * spl;
+    int flips = 0;
+    for (int s = 0; s < multi && launch_err == hipSuccess; ++s, ++flips)
+        launch_err = enqueue_multi(h, (b + flips) & 1);
+    for (int s = 0; s < singles && launch_err == hipSuccess; ++s, ++flips)
+        launch_err = enqueue_step(h, (b + flips) & 1, false);
     hipGraph_t g = nullptr;
     hipError_t end_err = hipStreamEndCapture(h->stream, &g);
     if (launch_err != hipSuccess || end_err != hipSuccess) {
@@ -233,9 +242,10 @@ int enqueue_run(lbm_handle* h, int nsteps, bool export_macro)
             h->cur ^= chunk_flips(h);  // one flip per launch: even for one-step chunks
         }
     }
-    if (two_step(h)) {
-        for (; plain >= 2; plain -= 2) {
-            HIP_TRY(enqueue_step2(h, h->cur));
+    const int spl = launch_steps(h);
+    if (spl > 1) {
+        for (; plain >= spl; plain -= spl) {
+            HIP_TRY(enqueue_multi(h, h->cur));
             h->cur ^= 1;
         }
     }
@@ -797,17 +807,20 @@ int lbm_autotune(lbm_handle* h, int steps, int* best_variant, double* ms_per_ste
     double best_ms = 1e300;
     for (int v = 0; v < kNumStepVariants && e == hipSuccess; ++v) {
         if (!step_variant_ok(v, h->d.nx, h->d.ny, h->d.x_edge)) continue;
-        const bool ts = step_variant_two_step(v);
+        const int per = step_variant_steps(v);  // steps per launch
         auto launch = [&]() {
-            hipError_t r = ts ? launch_phys_step2(v, h->f16, h->d.x_edge, h->force(), h->phys(), h->buf[h->cur],
-                                                  h->buf[h->cur ^ 1], h->bits, h->words, h->stream)
-                              : launch_phys_step(v, h->f16, h->d.x_edge, h->force(), false, h->phys(),
-                                                 h->buf[h->cur], h->buf[h->cur ^ 1], h->bits, h->words, h->macro,
-                                                 h->stream);
+            hipError_t r = per == 3   ? launch_phys_step3(v, h->f16, h->d.x_edge, h->force(), h->phys(),
+                                                          h->buf[h->cur], h->buf[h->cur ^ 1], h->bits, h->words,
+                                                          h->stream)
+                           : per == 2 ? launch_phys_step2(v, h->f16, h->d.x_edge, h->force(), h->phys(),
+                                                          h->buf[h->cur], h->buf[h->cur ^ 1], h->bits, h->words,
+                                                          h->stream)
+                                      : launch_phys_step(v, h->f16, h->d.x_edge, h->force(), false, h->phys(),
+                                                         h->buf[h->cur], h->buf[h->cur ^ 1], h->bits, h->words,
+                                                         h->macro, h->stream);
             h->cur ^= 1;
             return r;
         };
-        const int per = ts ? 2 : 1;  // steps per launch
         for (int s = 0; s < 2 && e == hipSuccess; ++s) e = launch();
         if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
         const int launches = (steps + per - 1) / per;

@@ -88,6 +88,7 @@ enum : int {
     kShfl = 8,     // aligned vector loads, x-neighbours from the adjacent lane
     kTwoStep = 16, // two steps per launch through an LDS tile (temporal blocking)
     kGridStride = 32, // batched: block b takes chunks b, b + grid, ... (else a contiguous range)
+    kThreeStep = 64,  // three steps per launch through an LDS tile with a two-cell ring
     kRowStream = 128  // two-step: strips of block-size columns streamed down a chunk of rows
 };
 

@@ -16,16 +16,23 @@ struct StepVariant {
     int block;             // threads per block
     int cells_per_thread;  // cells handled by one thread
     int vec;               // consecutive cells per lane (1: strided by block)
-    int flags;             // kNtStore | kNtLoad | kXcdRemap | kShfl | kTwoStep
+    int flags;             // kNtStore | kNtLoad | kXcdRemap | kShfl | kTwoStep | kThreeStep | kRowStream
 };
-// kTwoStep variants: cells_per_thread = tile rows, vec = tile width; one
-// launch advances two steps (launch_phys_step2), and single steps use the
-// one-step variant kSingleStepFallback.
+// kTwoStep and kThreeStep variants: cells_per_thread = tile (or chunk)
+// rows, vec = tile (or strip) width; one launch advances two steps
+// (launch_phys_step2) or three (launch_phys_step3), and the steps left over
+// use the one-step variant kSingleStepFallback.
 constexpr int kSingleStepFallback = 6;
 extern const StepVariant kStepVariants[];
 extern const int kNumStepVariants;
 bool step_variant_ok(int variant, int nx, int ny, int xedge);
 inline bool step_variant_two_step(int variant) { return (kStepVariants[variant].flags & kTwoStep) != 0; }
+// Steps one launch of a variant advances: 1, 2 (kTwoStep) or 3 (kThreeStep).
+inline int step_variant_steps(int variant)
+{
+    const int f = kStepVariants[variant].flags;
+    return (f & kThreeStep) ? 3 : ((f & kTwoStep) ? 2 : 1);
+}
 // Whether two-step tiles of width bw fit an nx x ny lattice (lbm_step2.hip).
 bool two_step_fits(int bw, int nx, int ny, int xedge);
 // Blocks of bs threads for n work items.
@@ -39,6 +46,8 @@ hipError_t launch_phys_step(int variant, bool f16, int xedge, bool force, bool e
                             const void* src, void* dst, const uint32_t* bits, uint32_t words, float* macro,
                             hipStream_t s);
 hipError_t launch_phys_step2(int variant, bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
+                             void* dst, const uint32_t* bits, uint32_t words, hipStream_t s);
+hipError_t launch_phys_step3(int variant, bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
                              void* dst, const uint32_t* bits, uint32_t words, hipStream_t s);
 hipError_t launch_phys_macro(bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
                              const uint8_t* mat, float* rho, float* ux, float* uy, hipStream_t s);

@@ -785,6 +785,12 @@ const StepVariant kStepVariants[] = {
     {1024, 128, 1024, kNtStore | kTwoStep | kRowStream | kXcdRemap},
     {512, 256, 512, kNtStore | kTwoStep | kRowStream | kXcdRemap},
     {256, 128, 256, kNtStore | kTwoStep | kRowStream | kXcdRemap},
+    // three steps per launch: (block, tile rows, tile width, flags)
+    {512, 16, 64, kNtStore | kThreeStep},
+    {512, 16, 64, kNtStore | kThreeStep | kXcdRemap},
+    {512, 8, 128, kNtStore | kThreeStep},
+    {256, 16, 32, kNtStore | kThreeStep},
+    {256, 8, 64, kNtStore | kThreeStep},
 };
 const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
 
@@ -792,7 +798,7 @@ bool step_variant_ok(int v, int nx, int ny, int xedge)
 {
     if (v < 0 || v >= kNumStepVariants) return false;
     const StepVariant& sv = kStepVariants[v];
-    if (sv.flags & kTwoStep) return two_step_fits(sv.vec, nx, ny, xedge);
+    if (sv.flags & (kTwoStep | kThreeStep)) return two_step_fits(sv.vec, nx, ny, xedge);
     if (sv.flags & kShfl) return xedge == 0 && nx % (sv.block * sv.vec) == 0;
     return sv.vec == 1 || nx % sv.vec == 0;
 }

@@ -234,8 +234,9 @@ def variant_shapes() -> list[tuple[int, int, int, int]]:
     flags: 1 non-temporal stores, 2 non-temporal loads, 4 rows per XCD,
     8 aligned vector loads with x-neighbours from the adjacent lane, 16 two
     steps per launch through an LDS tile of cells_per_thread rows x vec
-    cells, 64 the tile's step-1 loads all issued first, 128 the two steps
-    streamed down strips of vec columns in chunks of cells_per_thread rows)."""
+    cells, 64 three steps per launch through an LDS tile of cells_per_thread
+    rows x vec cells with a two-cell ring, 128 the two steps streamed down
+    strips of vec columns in chunks of cells_per_thread rows)."""
     lib = load_library()
     out = []
     for v in range(lib.lbm_variant_count()):
@@ -247,12 +248,12 @@ def variant_shapes() -> list[tuple[int, int, int, int]]:
 
 def variant_ok(variant: int, nx: int, ny: int, x_edge: int = X_PERIODIC) -> bool:
     """Whether a tile variant can step an nx x ny lattice (the library's
-    own rule: two-step variants need nx, ny >= 3 and, with the inlet/outlet
-    edge, a last tile of at least two columns (nx % vec != 1); lane-shuffle
+    own rule: two- and three-step variants need nx, ny >= 3 and, with the
+    inlet/outlet edge, a last tile of at least two columns (nx % vec != 1); lane-shuffle
     variants need periodic x and whole blocks of block*vec cells per row;
     vector variants need nx % vec == 0)."""
     b, c, v, f = variant_shapes()[variant]
-    if f & 16:
+    if f & (16 | 64):
         return nx >= 3 and ny >= 3 and (x_edge == X_PERIODIC or nx % v != 1)
     if f & 8:
         return x_edge == X_PERIODIC and nx % (b * v) == 0

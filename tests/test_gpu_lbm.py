@@ -90,9 +90,10 @@ def test_tiny_periodic_lattices_against_oracles(nx, ny):
     one-cell ring wrap around the whole lattice: parity mode bit for bit
     against the WGSL restatement (both buffers, every slot: no solids), and
     physics mode with a body force (and a solid cell when the lattice has
-    room), with the default one-step shape, the 128 x 8 two-step tile and
-    the 512-column two-step row stream, against the binary64 oracle; after
-    1, 2, 7 and 100 steps."""
+    room), with the default one-step shape, the 128 x 8 two-step tile, the
+    512-column two-step row stream and the 64 x 16 three-step tile (its
+    two-cell ring wraps around the lattice more than once here), against
+    the binary64 oracle; after 1, 2, 7 and 100 steps."""
     from test_oracle_wgsl import bulk_info
     print(f"case: tiny lattice {nx}x{ny}", flush=True)
     rng = np.random.default_rng(100 * nx + ny)
@@ -112,7 +113,7 @@ def test_tiny_periodic_lattices_against_oracles(nx, ny):
         st = O.PhysState(O.phys_params(nx, ny, 0.7, force=force), solid, f0)
         st.run(steps)
         r0, u0, v0 = st.macro()
-        for variant in (None, 18, 34):
+        for variant in (None, 18, 34, 41):
             lat = L.Lattice(nx, ny, 0.7, force=force)
             lat.write_materials(W.materials_from_solid(solid))
             lat.write_populations(f0.astype(np.float32), 0)
@@ -374,6 +375,7 @@ def test_tile_variants_and_graphs_bit_identical(storage, nx, ny, x_edge):
         assert any(n[2][3] & 8 for n in names), "no lane-shuffle variant ran"
     assert sum(1 for n in names if n[2][3] & 16) >= 2 * 8, "two-step variants did not run"
     assert sum(1 for n in names if n[2][3] & 128) >= 2 * 4, "row-stream variants did not run"
+    assert sum(1 for n in names if n[2][3] & 64) >= 2 * 4, "three-step variants did not run"
     assert np.isfinite(outs[0]).all()
     vis = visible_mask(solid, periodic_x=not x_edge)
     for k, o in enumerate(outs[1:], 1):
