@@ -246,7 +246,11 @@ int lbm_variant_count(void);
  * adjacent lane (periodic x, nx a multiple of block*vec), bit 4 two steps
  * per launch through an LDS tile of cells_per_thread rows x vec cells
  * (nx, ny >= 3; with LBM_X_INLET_OUTLET, nx % vec != 1; odd step counts
- * end with a one-step launch). */
+ * end with a one-step launch), bit 6 three steps per launch through an LDS
+ * tile with a two-cell ring (same conditions; the steps left over run one
+ * per launch), bit 7 (with bit 4) the two steps streamed down strips of vec
+ * columns in chunks of cells_per_thread rows.  Every variant gives the same
+ * bits at every slot a fluid cell can read. */
 int lbm_variant_info(int variant, int* block, int* cells_per_thread, int* vec, int* flags);
 int lbm_set_variant(lbm_handle* h, int variant);
 int lbm_get_variant(lbm_handle* h, int* variant);
