@@ -233,7 +233,7 @@ def run_batch(args):
         "dtype": "f16-deviation storage, f32 math" if args.f16 else "f32",
         "data": "synthetic",
         "config": {"workload": workload, "domains": len(doms), "cells": int(cells), "variant": variant,
-                   "batch_variant": {"kernel": "k_batch_step2" if per_launch == 2 else "k_batch_step",
+                   "batch_variant": {"kernel": {3: "k_batch_step3", 2: "k_batch_step2"}.get(per_launch, "k_batch_step"),
                                      "block": block, "vec": vec, "tile_rows": tile_h, "flags": flags},
                    "graph_chunk": args.graph_chunk, "seed": W.SEED,
                    "finite_domains": len(doms) - len(diverged), "diverged_domains": diverged,

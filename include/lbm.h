@@ -303,10 +303,11 @@ int lbm_batch_macro(lbm_batch* b, int domain, float* rho, float* ux, float* uy);
 int lbm_batch_reduce(lbm_batch* b, int domain, double out[5]);
 int lbm_batch_variant_count(void);
 /* block: threads per block; vec: consecutive cells per lane, or the tile
- * width of a two-step variant; tile_h: tile rows of a two-step variant (0
- * otherwise); flags: bit 0 non-temporal stores, bit 2 work grouped per
- * XCD, bit 4 two steps per launch (one block per tile), bit 5 persistent
- * blocks take every grid-th chunk. */
+ * width of a two- or three-step variant; tile_h: tile rows of such a
+ * variant (0 otherwise); flags: bit 0 non-temporal stores, bit 2 work
+ * grouped per XCD, bit 4 two steps per launch (one block per tile), bit 5
+ * persistent blocks take every grid-th chunk, bit 6 three steps per launch
+ * (one block per tile, a two-cell ring). */
 int lbm_batch_variant_info(int variant, int* block, int* vec, int* tile_h, int* flags);
 int lbm_batch_set_variant(lbm_batch* b, int variant);
 int lbm_batch_get_variant(lbm_batch* b, int* variant);

@@ -47,7 +47,7 @@ struct lbm_batch {
     BatchDomainDev* ddom = nullptr;
     uint16_t* chunk_dom = nullptr;  // chunk -> domain for the one-step variant in use
     uint32_t nchunks = 0;
-    uint16_t* tile_dom = nullptr;   // tile -> domain for a two-step variant
+    uint16_t* tile_dom = nullptr;   // tile -> domain for a two- or three-step variant
     uint32_t ntiles = 0;
     float* tmp3 = nullptr;
     double* partial = nullptr;
@@ -61,9 +61,10 @@ struct lbm_batch {
 
     PhysArgs args(int d) const { return dom[d].a; }
     void* view(int b, int d) const { return (char*)buf[b] + (size_t)dom[d].pop_off * elem; }
-    bool two_step() const { return batch_variant_two_step(variant); }
-    // the variant of single steps: the chosen one, or the fallback of a two-step one
-    int step1_variant() const { return two_step() ? kBatchSingleStepFallback : variant; }
+    // steps one launch of the chosen variant advances (1, 2 or 3)
+    int steps() const { return batch_variant_steps(variant); }
+    // the variant of single steps: the chosen one, or the fallback of a multi-step one
+    int step1_variant() const { return steps() > 1 ? kBatchSingleStepFallback : variant; }
     uint32_t grid() const
     {
         const uint32_t g = (uint32_t)(kBatchVariants[step1_variant()].blocks_per_cu * cus);
@@ -118,8 +119,8 @@ int upload_table(lbm_batch* b, const std::vector<uint16_t>& table, uint16_t** de
 }
 
 // Work tables of the current variant: chunks of block*vec cells for the
-// one-step variant (the chosen one or the two-step fallback), and tiles for
-// a two-step variant.
+// one-step variant (the chosen one or the multi-step fallback), and tiles
+// for a two- or three-step variant.
 int build_chunks(lbm_batch* b)
 {
     const BatchVariant& sv = kBatchVariants[b->step1_variant()];
@@ -130,7 +131,7 @@ int build_chunks(lbm_batch* b)
         D.chunk0 = (uint32_t)table.size();
         table.insert(table.end(), (D.ncells + w - 1) / w, (uint16_t)d);
         D.tile0 = (uint32_t)tiles.size();
-        if (b->two_step()) {
+        if (b->steps() > 1) {
             const BatchVariant& tv = kBatchVariants[b->variant];
             const uint32_t nt = (uint32_t)((D.a.nx + tv.vec - 1) / tv.vec) * (uint32_t)((D.a.ny + tv.tile_h - 1) / tv.tile_h);
             tiles.insert(tiles.end(), nt, (uint16_t)d);
@@ -151,7 +152,7 @@ int build_chunks(lbm_batch* b)
 bool batch_variant_ok(const lbm_batch* b, int v)
 {
     if (v < 0 || v >= kNumBatchVariants) return false;
-    if (!batch_variant_two_step(v)) return true;
+    if (batch_variant_steps(v) == 1) return true;
     for (const lbm_domain_desc& q : b->desc)
         if (q.x_edge != LBM_X_PERIODIC && q.nx % kBatchVariants[v].vec == 1) return false;
     return true;
@@ -164,26 +165,35 @@ hipError_t batch_enqueue_step(lbm_batch* b, int swap)
                              b->buf[swap], b->buf[swap ^ 1], b->bits, b->stream);
 }
 
-// Two steps from buffer `swap` into buffer 1-swap (two-step variant).
-hipError_t batch_enqueue_step2(lbm_batch* b, int swap)
+// b->steps() steps from buffer `swap` into buffer 1-swap (multi-step variant).
+hipError_t batch_enqueue_multi(lbm_batch* b, int swap)
 {
+    if (b->steps() == 3)
+        return launch_batch_step3(b->variant, b->f16, b->ddom, b->tile_dom, b->ntiles, b->buf[swap],
+                                  b->buf[swap ^ 1], b->bits, b->stream);
     return launch_batch_step2(b->variant, b->f16, b->ddom, b->tile_dom, b->ntiles, b->buf[swap], b->buf[swap ^ 1],
                               b->bits, b->stream);
 }
 
-// Buffer flips of one graph chunk: one per launch.
-int batch_chunk_flips(const lbm_batch* b) { return b->two_step() ? (b->graph_chunk / 2) & 1 : 0; }
+// Buffer flips of one graph chunk: one per launch (its multi-step launches,
+// then one-step launches for the steps left over).
+int batch_chunk_flips(const lbm_batch* b)
+{
+    const int spl = b->steps();
+    return (b->graph_chunk / spl + b->graph_chunk % spl) & 1;
+}
 
 int batch_ensure_graph(lbm_batch* b, int start)
 {
     if (b->graph[start]) return LBM_OK;
     BATCH_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t le = hipSuccess;
-    if (b->two_step()) {
-        for (int s = 0; s < b->graph_chunk / 2 && le == hipSuccess; ++s) le = batch_enqueue_step2(b, (start + s) & 1);
-    } else {
-        for (int s = 0; s < b->graph_chunk && le == hipSuccess; ++s) le = batch_enqueue_step(b, (start + s) & 1);
-    }
+    // multi-step launches, then one-step launches for the steps left over
+    const int spl = b->steps();
+    const int multi = spl > 1 ? b->graph_chunk / spl : 0, singles = b->graph_chunk - multi * spl;
+    int flips = 0;
+    for (int s = 0; s < multi && le == hipSuccess; ++s, ++flips) le = batch_enqueue_multi(b, (start + flips) & 1);
+    for (int s = 0; s < singles && le == hipSuccess; ++s, ++flips) le = batch_enqueue_step(b, (start + flips) & 1);
     hipGraph_t g = nullptr;
     hipError_t ee = hipStreamEndCapture(b->stream, &g);
     if (le != hipSuccess || ee != hipSuccess) {
@@ -211,9 +221,10 @@ int batch_enqueue_run(lbm_batch* b, int nsteps)
             b->cur ^= batch_chunk_flips(b);  // one flip per launch: even for one-step chunks
         }
     }
-    if (b->two_step()) {
-        for (; left >= 2; left -= 2) {
-            BATCH_TRY(batch_enqueue_step2(b, b->cur));
+    const int spl = b->steps();
+    if (spl > 1) {
+        for (; left >= spl; left -= spl) {
+            BATCH_TRY(batch_enqueue_multi(b, b->cur));
             b->cur ^= 1;
         }
     }
@@ -567,10 +578,9 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* best_variant, double* ms_pe
             e = hipErrorUnknown;
             break;
         }
-        const bool ts = b->two_step();
-        const int per = ts ? 2 : 1;  // steps per launch
+        const int per = b->steps();  // steps per launch
         auto launch = [&]() {
-            hipError_t r = ts ? batch_enqueue_step2(b, b->cur) : batch_enqueue_step(b, b->cur);
+            hipError_t r = per > 1 ? batch_enqueue_multi(b, b->cur) : batch_enqueue_step(b, b->cur);
             b->cur ^= 1;
             return r;
         };

@@ -82,21 +82,30 @@ hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s
 // counts take the one-step variant kBatchSingleStepFallback.
 struct BatchVariant {
     int block;           // threads per block
-    int vec;             // consecutive cells per lane; tile width when two-step
+    int vec;             // consecutive cells per lane; tile width when two- or three-step
     int blocks_per_cu;   // persistent grid = blocks_per_cu * CUs (one-step)
-    int flags;           // kNtStore | kXcdRemap | kGridStride | kTwoStep
-    int tile_h;          // tile rows (two-step)
+    int flags;           // kNtStore | kXcdRemap | kGridStride | kTwoStep | kThreeStep
+    int tile_h;          // tile rows (two- and three-step)
 };
 constexpr int kBatchSingleStepFallback = 9;
 extern const BatchVariant kBatchVariants[];
 extern const int kNumBatchVariants;
 inline bool batch_variant_two_step(int variant) { return (kBatchVariants[variant].flags & kTwoStep) != 0; }
+// Steps one launch of a batch variant advances: 1, 2 (kTwoStep) or 3 (kThreeStep).
+inline int batch_variant_steps(int variant)
+{
+    const int f = kBatchVariants[variant].flags;
+    return (f & kThreeStep) ? 3 : ((f & kTwoStep) ? 2 : 1);
+}
 hipError_t launch_batch_step(int variant, bool f16, const BatchDomainDev* doms, const uint16_t* chunk_dom,
                              uint32_t nchunks, uint32_t grid, const void* src, void* dst, const uint32_t* bits,
                              hipStream_t s);
 // Two steps of every domain: tile_dom[t] is the domain of tile t, domain d
 // owns tiles [tile0, tile0 + ceil(nx/vec) * ceil(ny/tile_h)).
 hipError_t launch_batch_step2(int variant, bool f16, const BatchDomainDev* doms, const uint16_t* tile_dom,
+                              uint32_t ntiles, const void* src, void* dst, const uint32_t* bits, hipStream_t s);
+// Three steps of every domain, with the tiles of a three-step variant.
+hipError_t launch_batch_step3(int variant, bool f16, const BatchDomainDev* doms, const uint16_t* tile_dom,
                               uint32_t ntiles, const void* src, void* dst, const uint32_t* bits, hipStream_t s);
 
 // Learned collision, restriction and health check.

@@ -497,6 +497,10 @@ const BatchVariant kBatchVariants[] = {
     {256, 32, 0, kNtStore | kTwoStep | kXcdRemap, 16}, {1024, 64, 0, kNtStore | kTwoStep | kXcdRemap, 16},
     {512, 64, 0, kNtStore | kTwoStep, 32}, {512, 64, 0, kNtStore | kTwoStep | kXcdRemap, 32},
     {1024, 64, 0, kNtStore | kTwoStep | kXcdRemap, 32},
+    // three steps per launch (launch_batch_step3): (block, tile width, -, flags, tile rows)
+    {512, 64, 0, kNtStore | kThreeStep, 16}, {512, 64, 0, kNtStore | kThreeStep | kXcdRemap, 16},
+    {256, 32, 0, kNtStore | kThreeStep, 16}, {256, 32, 0, kNtStore | kThreeStep | kXcdRemap, 16},
+    {256, 64, 0, kNtStore | kThreeStep, 8},
 };
 const int kNumBatchVariants = (int)(sizeof(kBatchVariants) / sizeof(kBatchVariants[0]));
 

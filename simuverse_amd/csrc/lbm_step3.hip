@@ -225,4 +225,60 @@ hipError_t launch_phys_step3(int variant, bool f16, int xedge, bool force, const
 #undef LBM_D3
 }
 
+// ---------------------------------------------------------------------------
+// Batched domains, three steps per launch: one block per tile of one domain
+// (tile_dom[t]; domain d owns tiles [tile0, tile0 + ceil(nx/BW) ceil(ny/BH))),
+// each domain with its own size, tau, inlet, force and x edge.
+// ---------------------------------------------------------------------------
+template <typename T, int BS, int BW, int BH, int FLAGS>
+__global__ __launch_bounds__(BS) void k_batch_step3(const BatchDomainDev* __restrict__ doms,
+                                                    const uint16_t* __restrict__ tile_dom, const T* __restrict__ src,
+                                                    T* __restrict__ dst, const uint32_t* __restrict__ bits)
+{
+    __shared__ T lds[9][kRegion<BW, BH, 2>];
+    const uint32_t t = (FLAGS & kXcdRemap) ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const BatchDomainDev D = doms[tile_dom[t]];
+    const uint32_t local = t - D.tile0;
+    const uint32_t tiles_x = ((uint32_t)D.a.nx + (uint32_t)(BW - 1)) / (uint32_t)BW;
+    const int x0 = (int)(local % tiles_x) * BW, y0 = (int)(local / tiles_x) * BH;
+    const T* s = src + D.pop_off;
+    T* o = dst + D.pop_off;
+    const uint32_t* m = bits + D.bits_off;
+    constexpr int F = FLAGS & ~kXcdRemap;
+    if (D.x_edge) {
+        if (D.force) three_step_tile<T, 1, true, BS, BW, BH, F>(D.a, s, o, m, D.words, x0, y0, lds);
+        else three_step_tile<T, 1, false, BS, BW, BH, F>(D.a, s, o, m, D.words, x0, y0, lds);
+    } else {
+        if (D.force) three_step_tile<T, 0, true, BS, BW, BH, F>(D.a, s, o, m, D.words, x0, y0, lds);
+        else three_step_tile<T, 0, false, BS, BW, BH, F>(D.a, s, o, m, D.words, x0, y0, lds);
+    }
+}
+
+template <typename T>
+static hipError_t launch_batch3_typed(int v, const BatchDomainDev* doms, const uint16_t* tile_dom, uint32_t ntiles,
+                                      const void* src, void* dst, const uint32_t* bits, hipStream_t s)
+{
+    if (ntiles == 0) return hipSuccess;
+#define LBM_BATCH3(BS, BW, BH, FL)                                                                             \
+    hipLaunchKernelGGL((k_batch_step3<T, BS, BW, BH, FL>), dim3(ntiles), dim3(BS), 0, s, doms, tile_dom,    \
+                       (const T*)src, (T*)dst, bits)
+    switch (v) {
+    case 23: LBM_BATCH3(512, 64, 16, kNtStore); break;
+    case 24: LBM_BATCH3(512, 64, 16, kNtStore | kXcdRemap); break;
+    case 25: LBM_BATCH3(256, 32, 16, kNtStore); break;
+    case 26: LBM_BATCH3(256, 32, 16, kNtStore | kXcdRemap); break;
+    case 27: LBM_BATCH3(256, 64, 8, kNtStore); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef LBM_BATCH3
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_step3(int variant, bool f16, const BatchDomainDev* doms, const uint16_t* tile_dom,
+                              uint32_t ntiles, const void* src, void* dst, const uint32_t* bits, hipStream_t s)
+{
+    return f16 ? launch_batch3_typed<_Float16>(variant, doms, tile_dom, ntiles, src, dst, bits, s)
+               : launch_batch3_typed<float>(variant, doms, tile_dom, ntiles, src, dst, bits, s);
+}
+
 }  // namespace lbm

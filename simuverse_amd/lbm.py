@@ -764,10 +764,11 @@ def batch_variant_count() -> int:
 
 
 def batch_variant_shapes() -> list[tuple[int, int, int, int]]:
-    """(threads per block, consecutive cells per lane or two-step tile
-    width, two-step tile rows (0 for one-step variants), flags: 1
+    """(threads per block, consecutive cells per lane or multi-step tile
+    width, multi-step tile rows (0 for one-step variants), flags: 1
     non-temporal stores, 4 work grouped per XCD, 16 two steps per launch,
-    32 persistent blocks take every grid-th chunk)."""
+    32 persistent blocks take every grid-th chunk, 64 three steps per
+    launch)."""
     lib = load_library()
     out = []
     for v in range(lib.lbm_batch_variant_count()):

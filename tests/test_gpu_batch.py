@@ -56,13 +56,13 @@ def gfx950():
 
 @pytest.mark.parametrize("storage", [L.STORAGE_F32, L.STORAGE_F16_DEV])
 def test_batch_domains_equal_single_domain_runs(storage):
-    """Every batch variant, one step per launch (persistent chunks) or two
-    (one block per LDS tile, tiles cut by the domain edges), with 100-step
-    graphs or direct launches and an odd step count (the last step of a
-    two-step variant is a one-step launch), equals single-domain runs.
-    One-step variants are compared at every slot; two-step variants at every
-    slot a fluid cell can read (a two-step launch leaves older values in
-    the solid-cell slots no fluid cell pulls from)."""
+    """Every batch variant, one step per launch (persistent chunks), two or
+    three (one block per LDS tile, tiles cut by the domain edges), with
+    100-step graphs or direct launches and a step count (137) that leaves
+    steps over for one-step launches, equals single-domain runs.  One-step
+    variants are compared at every slot; multi-step variants at every slot
+    a fluid cell can read (a multi-step launch leaves older values in the
+    solid-cell slots no fluid cell pulls from)."""
     print(f"case: batch of 6 vs single, storage {storage}", flush=True)
     doms = [dict(nx=nx, ny=ny, tau=tau, x_edge=xe, inlet_ux=ux, force=fo) for nx, ny, tau, xe, ux, fo in SPECS]
     setups = [domain_setup(k, s[0], s[1], s[3]) for k, s in enumerate(SPECS)]
@@ -71,8 +71,9 @@ def test_batch_domains_equal_single_domain_runs(storage):
     vis = [visible_mask(setups[k][0], periodic_x=s[3] == 0) for k, s in enumerate(SPECS)]
     shapes = L.batch_variant_shapes()
     assert sum(1 for sh in shapes if sh[3] & 16) >= 4, "no two-step batch variants"
+    assert sum(1 for sh in shapes if sh[3] & 64) >= 4, "no three-step batch variants"
     for v in range(L.batch_variant_count()):
-        two_step = bool(shapes[v][3] & 16)
+        two_step = bool(shapes[v][3] & (16 | 64))
         b = L.Batch(doms, storage=storage)
         for k, (solid, f0) in enumerate(setups):
             b.write_materials(k, W.materials_from_solid(solid))
