@@ -177,6 +177,22 @@ __global__ __launch_bounds__(BS) void k_phys_step2(PhysArgs a, const T* __restri
     two_step_tile<T, XEDGE, FORCE, BS, BW, BH, FLAGS & ~kXcdRemap>(a, src, dst, bits, words, x0, y0, lds);
 }
 
+// a if pick, else b, as a bit merge under a mask the compiler cannot see
+// through: both values stay separate LDS reads at fixed offsets from one
+// address, instead of one read from a selected address per direction,
+// which the compiler would then share with the plain pulls (nine address
+// additions for every cell; with the merge, three, one per ring slot).
+template <typename T>
+__device__ __forceinline__ T lds_pick(bool pick, T a, T b)
+{
+    static_assert(sizeof(T) == 2 || sizeof(T) == 4, "16- or 32-bit element");
+    using U = typename std::conditional<sizeof(T) == 2, uint16_t, uint32_t>::type;
+    uint32_t m = pick ? 0xFFFFFFFFu : 0u;
+    asm volatile("" : "+v"(m));
+    const uint32_t ua = (uint32_t)__builtin_bit_cast(U, a), ub = (uint32_t)__builtin_bit_cast(U, b);
+    return __builtin_bit_cast(T, (U)((ua & m) | (ub & ~m)));
+}
+
 // Two steps per launch, streamed down a strip of rows.  A block of BS
 // threads owns the columns [x0, x0 + BS) of the rows [y0, y0 + H) (the last
 // strip and chunk may be cut by the lattice edge) and walks down them one
@@ -296,7 +312,8 @@ __global__ __launch_bounds__(BS) void k_phys_step2_rows(PhysArgs a, const T* __r
                 const uint32_t srow = ey == 1 ? gb.sm : (ey == -1 ? gb.sp : gb.s0);
                 bool own = (srow >> (1 - ex)) & 1u;
                 if (XEDGE == 1 && ex != 0) own = own || cx - ex < 0 || cx - ex >= nx;
-                const T v = own ? lds[(j + 1) & 3][inv_of(i)][c] : lds[(j + 1 - ey) & 3][i][c - ex];
+                const T pulled = lds[(j + 1 - ey) & 3][i][c - ex];
+                const T v = i == 0 ? pulled : lds_pick(own, lds[(j + 1) & 3][inv_of(i)][c], pulled);
                 g[i] = Dom<T, SC>::in(v, a.inv_scale);
             }
             if (XEDGE == 1 && x == 0) zou_he_inlet<SC>(a, g);
