@@ -334,7 +334,7 @@ def run_learned(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 lattice, bf16 network on matrix cores (f32 accumulate)",
+        "dtype": "f32 lattice, bf16 network operands on matrix cores (f32 accumulate)",
         "data": "synthetic",
         "config": {
             "workload": f"d2q9_learned_{n}_to_{nc}",
@@ -349,7 +349,9 @@ def run_learned(args):
                                 "gain": r.gain, "vel_l2": r.gated_err, "coarse_ms": round(r.gated_ms, 3),
                                 "speedup_vs_fine": round(r.fine_ms / r.gated_ms, 2)},
             "vel_l2_vs_restricted_fine": {"plain": r.plain_err, "learned": r.learned_err},
-            "coarse_ms": {"plain": round(r.plain_ms, 3), "learned": round(r.learned_ms, 3)},
+            "coarse_ms": {"plain": round(r.plain_ms, 3), "learned": round(r.learned_ms, 3),
+                          "learned_binary16_operands": round(r.learned_f16_ms, 3)},
+            "vel_l2_learned_binary16_operands": r.learned_f16_err,
             "time_to_solution_speedup": {"learned_vs_fine": round(r.fine_ms / r.learned_ms, 2),
                                          "learned_incl_training_vs_fine":
                                              round(r.fine_ms / (r.learned_ms + 1e3 * (r.collect_s + r.train_s)), 3)},

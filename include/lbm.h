@@ -345,10 +345,10 @@ int lbm_mlp_get_scales(lbm_mlp* m, float* in_scale, float* out_scale);
 /* The raw network on n samples (x: n*9 floats, y: n*9 floats, host). */
 int lbm_mlp_forward(lbm_mlp* m, const float* x, int n, float* y);
 /* The same on the matrix cores: inputs, weights and hidden activations
- * rounded to bf16, binary32 accumulation (LBM_MLP_BF16). */
-int lbm_mlp_forward_bf16(lbm_mlp* m, const float* x, int n, float* y);
-/* The same with binary16 operands and GELU (LBM_MLP_F16, the learned
+ * rounded to bf16, binary32 accumulation (LBM_MLP_BF16, the learned
  * step's default). */
+int lbm_mlp_forward_bf16(lbm_mlp* m, const float* x, int n, float* y);
+/* The same with binary16 operands and GELU (LBM_MLP_F16). */
 int lbm_mlp_forward_f16(lbm_mlp* m, const float* x, int n, float* y);
 /* Loss 1/(9n) sum (y - t)^2 of n host samples and its gradient (5385). */
 int lbm_mlp_loss_grad(lbm_mlp* m, const float* x, const float* t, int n, float* grad, double* loss);
@@ -400,10 +400,10 @@ int lbm_mlp_train(lbm_mlp* m, int steps, int batch, float lr, uint64_t seed, dou
  * parameter updates first wait for all work queued on the device, and
  * handles re-capture their HIP-graph chunks. */
 int lbm_set_learned_collision(lbm_handle* h, const lbm_mlp* m);
-/* Arithmetic of the learned step's network: LBM_MLP_F16 (matrix cores,
- * binary16 operands and a binary16 GELU, binary32 accumulation; default),
- * LBM_MLP_BF16 (matrix cores, bf16 operands) or LBM_MLP_F32 (binary32 on
- * the vector ALUs). */
+/* Arithmetic of the learned step's network: LBM_MLP_BF16 (matrix cores,
+ * bf16 operands, binary32 accumulation; the default), LBM_MLP_F16 (matrix
+ * cores, binary16 operands and a binary16 GELU: faster and closer to
+ * binary32) or LBM_MLP_F32 (binary32 on the vector ALUs). */
 enum { LBM_MLP_BF16 = 0, LBM_MLP_F32 = 1, LBM_MLP_F16 = 2 };
 int lbm_set_learned_precision(lbm_handle* h, int precision);
 /* Largest |u| over fluid cells of the current state and whether every

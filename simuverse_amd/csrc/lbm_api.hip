@@ -90,7 +90,7 @@ struct lbm_handle {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     const lbm_mlp* mlp = nullptr;   // learned collision correction, if set
     uint64_t graph_mlp_gen = 0;     // the network's generation the graphs were captured with
-    int mlp_precision = LBM_MLP_F16;
+    int mlp_precision = LBM_MLP_BF16;
     uint32_t* health = nullptr;     // [max |u|^2 bits, non-finite flag]
     void* backup = nullptr;         // chunk start state of lbm_run_guarded
 

@@ -398,8 +398,8 @@ class Lattice:
         self._net = net  # keep the network alive while attached
 
     def set_learned_precision(self, precision: int) -> None:
-        """MLP_F16 (matrix cores, binary16 operands and GELU; the default),
-        MLP_BF16 (matrix cores, bf16 operands) or MLP_F32 (vector ALUs)."""
+        """MLP_BF16 (matrix cores, bf16 operands; the default), MLP_F16
+        (matrix cores, binary16 operands and GELU) or MLP_F32 (vector ALUs)."""
         _check(self._lib.lbm_set_learned_precision(self._h, precision))
 
     def health(self) -> tuple[float, bool]:

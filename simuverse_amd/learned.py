@@ -19,7 +19,10 @@ Flow (n x n fine, (n/4)^2 coarse):
             at N_f/2 is kept
   coarse    N_f/4 coarse steps from the restricted initial state, plain BGK
             and BGK + the network at the chosen gain (timed), judged by the
-            velocity L2 error against the restricted fine state at N_f
+            velocity L2 error against the restricted fine state at N_f; the
+            network runs on the matrix cores with bf16 operands (the
+            library's default), and once more with binary16 operands for
+            comparison
 """
 from __future__ import annotations
 
@@ -54,6 +57,8 @@ class LearnedResult:
     plain_err: float = float("nan")
     learned_ms: float = 0.0
     learned_err: float = float("nan")
+    learned_f16_ms: float = 0.0
+    learned_f16_err: float = float("nan")
     coarse_steps: int = 0
     coarse_health: dict = field(default_factory=dict)
 
@@ -132,12 +137,13 @@ def run(n: int = 8192, fine_steps: int = 4000, force: float = 1e-7, pair_every: 
 
     fluid_c = ~solid_c
 
-    def coarse(learned: bool, steps: int, ref_u):
+    def coarse(learned: bool, steps: int, ref_u, precision: int = L.MLP_BF16):
         lat = L.Lattice(nc, nc, TAU_COARSE, force=gc)
         lat.write_materials(W.materials_from_solid(solid_c))
         lat.write_populations(fc0, 0)
         if learned:
             lat.set_learned_collision(net)
+            lat.set_learned_precision(precision)
         else:
             lat.autotune(steps=10)
             lat.write_populations(fc0, 0)
@@ -162,8 +168,10 @@ def run(n: int = 8192, fine_steps: int = 4000, force: float = 1e-7, pair_every: 
     # to plain BGK and the learned kernel is timed at gain 1 for reference
     net.set_scales(s_in, s_out * (res.gain if res.gain > 0.0 else 1.0))
     res.learned_ms, res.learned_err, h_learn = coarse(True, res.coarse_steps, (uxr, uyr))
+    res.learned_f16_ms, res.learned_f16_err, _ = coarse(True, res.coarse_steps, (uxr, uyr), L.MLP_F16)
     res.coarse_health = {"plain": h_plain, "learned": h_learn}
     net.close()
     log(f"coarse plain: {res.plain_ms:.1f} ms, vel L2 {res.plain_err:.4e}; learned (gain "
-        f"{res.gain if res.gain > 0 else 1:g}): {res.learned_ms:.1f} ms, vel L2 {res.learned_err:.4e}")
+        f"{res.gain if res.gain > 0 else 1:g}, bf16 operands): {res.learned_ms:.1f} ms, vel L2 "
+        f"{res.learned_err:.4e}; binary16 operands: {res.learned_f16_ms:.1f} ms, vel L2 {res.learned_f16_err:.4e}")
     return res

@@ -228,7 +228,7 @@ def test_learned_step_matrix_cores_track_f32():
     assert d_corr > 0
     assert d_bf16 < 5e-2 * d_corr
     assert d_f16 < 5e-2 * d_corr
-    assert np.array_equal(out["default"], out["f16"])
+    assert np.array_equal(out["default"], out["bf16"])
 
 
 def test_blow_up_falls_back_to_bgk():
