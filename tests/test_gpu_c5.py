@@ -2,9 +2,10 @@
 4x restriction) on the GPU:
 
 * the whole workflow of simuverse_amd/learned.py (fine run, transitions,
-  device training, validation gate against plain BGK, timed coarse runs)
-  on the same flow family at 1024^2 -> 256^2 (4x4 cylinders of radius
-  n/64, Re 240): every run finite, the gate's choice and errors recorded;
+  device training, validation gate against plain BGK, timed coarse runs
+  with bf16 and binary16 network operands) on the same flow family at
+  1024^2 -> 256^2 (4x4 cylinders of radius n/64, Re 240): every run
+  finite, the gate's choice and errors recorded;
 * the learned step at the configuration's full coarse size, 2048^2 with
   4x4 cylinders of radius 32: mass conserved, the momentum identity
   P(t+1) - P(t) = g N - drag step by step (the correction carries no
@@ -34,6 +35,7 @@ def test_c5_workflow_1024_to_256():
     print(json.dumps({"gate": {f"{k:g}": v for k, v in r.gate_errors.items()}, "gain": r.gain,
                       "plain_err": r.plain_err, "learned_err": r.learned_err, "gated_err": r.gated_err,
                       "fine_ms": r.fine_ms, "plain_ms": r.plain_ms, "learned_ms": r.learned_ms,
+                      "learned_f16_err": r.learned_f16_err, "learned_f16_ms": r.learned_f16_ms,
                       "samples": r.samples, "loss": r.final_loss}), flush=True)
     assert r.pairs == 8 and r.samples > 100_000
     assert np.isfinite(r.final_loss) and r.final_loss < 1.0  # targets normalised to unit rms
@@ -46,6 +48,10 @@ def test_c5_workflow_1024_to_256():
     # of the restricted fine solution on this resolved flow
     assert r.gated_err <= 1.25 * r.plain_err
     assert r.plain_err < 0.1 and r.learned_err < 0.1
+    # the two 16-bit operand types of the network give the same solution
+    # to within their rounding (bf16 is the default; binary16 is timed
+    # beside it)
+    assert abs(r.learned_f16_err - r.learned_err) < 0.1 * r.learned_err
 
 
 @pytest.mark.parametrize("precision", [L.MLP_BF16, L.MLP_F16, L.MLP_F32], ids=["bf16", "mma16", "f32"])
