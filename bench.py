@@ -87,6 +87,23 @@ def roofline(bytes_per_cell: int, cells: int, step_s: float, per_launch: int, tr
             "effective_GBps": round(effective, 1), "effective_frac": round(effective / HBM_PEAK_GBS, 4)}
 
 
+def rho_u_errors(pairs):
+    """Relative L2 and largest absolute errors of rho and |u - u_ref| over
+    the fluid cells of one or more (device, reference, fluid mask) triples,
+    each of (rho, ux, uy[, uz]) arrays."""
+    sr = sr0 = su = su0 = 0.0
+    mr = mu = umax = 0.0
+    for dev, ref, fl in pairs:
+        dr = (dev[0] - ref[0])[fl]
+        du = np.sqrt(sum((d - r)[fl] ** 2 for d, r in zip(dev[1:], ref[1:])))
+        uo = np.sqrt(sum(r[fl] ** 2 for r in ref[1:]))
+        sr += float(np.sum(dr ** 2)); sr0 += float(np.sum(ref[0][fl] ** 2))
+        su += float(np.sum(du ** 2)); su0 += float(np.sum(uo ** 2))
+        mr = max(mr, float(np.max(np.abs(dr)))); mu = max(mu, float(np.max(du))); umax = max(umax, float(np.max(uo)))
+    return {"rho_l2_rel": float(np.sqrt(sr / sr0)), "rho_max_abs": mr,
+            "u_l2_rel": float(np.sqrt(su / max(su0, 1e-300))), "u_max_abs": mu, "u_max": umax}
+
+
 def accuracy_vs_oracle(lat, solid, tau: float, force, steps: int = 20):
     """rho and u of the device after `steps` more steps from its current
     state, against the binary64 oracle run from the same populations: relative
@@ -113,6 +130,98 @@ def accuracy_vs_oracle(lat, solid, tau: float, force, steps: int = 20):
            "u_l2_rel": float(np.linalg.norm(du) / max(np.linalg.norm(uo), 1e-300)),
            "u_max_abs": float(np.max(du)), "u_max": float(np.max(uo)),
            "seconds": round(time.perf_counter() - t0, 1)}
+    return out
+
+
+def accuracy_batch(b, doms, finite, steps: int = 20, every: int = 8):
+    """Every `every`-th finite domain of a batch, `steps` more steps on the
+    device and by the binary64 oracle from the same populations: rho and u
+    errors over the sampled domains' fluid cells together."""
+    from oracle import oracle as O
+    from simuverse_amd import lbm as L
+
+    t0 = time.perf_counter()
+    sample = [k for k in finite[::every]]
+    states = {}
+    for k in sample:
+        d = doms[k]
+        states[k] = O.PhysState(O.phys_params(d.nx, d.ny, d.tau, x_edge=L.X_INLET_OUTLET, inlet_ux=d.inlet_ux),
+                                d.solid, b.read_populations(k))
+    b.run(steps)
+    pairs = []
+    for k in sample:
+        states[k].run(steps)
+        pairs.append((b.macro(k), states[k].macro(), ~doms[k].solid))
+        del states[k]
+    out = {"vs": "binary64 oracle (oracle/phys_oracle.c)", "steps": steps, "from": "the state after the timed run",
+           "domains": sample, "cells": int(sum(doms[k].nx * doms[k].ny for k in sample))}
+    out.update(rho_u_errors(pairs))
+    out["seconds"] = round(time.perf_counter() - t0, 1)
+    return out
+
+
+def accuracy_coarse(r, plain_steps: int = 20, learned_steps: int = 2):
+    """The coarse lattice of the learned workload from its restricted initial
+    state: the plain step (the gate's fallback) `plain_steps` steps and the
+    learned step with the trained network at the chosen gain (bf16 operands,
+    the default) `learned_steps` steps, each against the binary64 oracle
+    (oracle/phys_oracle.c, oracle/learned_oracle.c) from the same
+    populations."""
+    from oracle import oracle as O
+    from simuverse_amd import learned as LC
+    from simuverse_amd import lbm as L
+    from simuverse_amd import workloads as W
+
+    t0 = time.perf_counter()
+    nc = r.solid_c.shape[0]
+    fl = ~r.solid_c
+    params = O.phys_params(nc, nc, LC.TAU_COARSE, force=r.force_c)
+    out = {"vs": "binary64 oracles (oracle/phys_oracle.c, oracle/learned_oracle.c)",
+           "from": "the restricted initial state"}
+    lat = L.Lattice(nc, nc, LC.TAU_COARSE, force=r.force_c)
+    lat.write_materials(W.materials_from_solid(r.solid_c))
+    lat.write_populations(r.fc0, 0)
+    lat.run(plain_steps)
+    ps = O.PhysState(params, r.solid_c, r.fc0)
+    ps.run(plain_steps)
+    out["plain"] = {"steps": plain_steps, **rho_u_errors([(lat.macro(), ps.macro(), fl)])}
+    lat.close()
+    del ps
+    s_in, s_out = r.net_scales
+    net = L.Network(seed=1)
+    net.set_params(r.net_params)
+    net.set_scales(s_in, s_out)
+    lat = L.Lattice(nc, nc, LC.TAU_COARSE, force=r.force_c)
+    lat.write_materials(W.materials_from_solid(r.solid_c))
+    lat.write_populations(r.fc0, 0)
+    lat.set_learned_collision(net)
+    lat.run(learned_steps)
+    post = O.learned_run(params, r.solid_c, r.fc0, r.net_params, s_in, s_out, learned_steps)
+    ref = O.PhysState(params, r.solid_c, post).macro()
+    out["learned"] = {"steps": learned_steps, "network": "bf16 operands", "gain": r.net_gain,
+                      **rho_u_errors([(lat.macro(), ref, fl)])}
+    lat.close()
+    net.close()
+    out["seconds"] = round(time.perf_counter() - t0, 1)
+    return out
+
+
+def accuracy_d3q19(lat, scene, tau: float, force, steps: int = 5):
+    """The 3-D lattice after the timed run, `steps` more steps on the device
+    and by the binary64 numpy oracle (oracle/d3q19.py) from the same
+    populations: rho and u errors over the fluid cells."""
+    from oracle import d3q19 as D3
+
+    t0 = time.perf_counter()
+    st = D3.D3Q19State(tau, scene.solid, lat.read_populations(), force=force)
+    lat.run(steps)
+    st.run(steps)
+    rho_o, u_o = st.macro()
+    del st
+    rho_d, u_d = lat.macro()
+    out = {"vs": "binary64 oracle (oracle/d3q19.py)", "steps": steps, "from": "the state after the timed run"}
+    out.update(rho_u_errors([((rho_d, *u_d), (rho_o, *u_o), ~scene.solid)]))
+    out["seconds"] = round(time.perf_counter() - t0, 1)
     return out
 
 
@@ -217,6 +326,10 @@ def run_batch(args):
     # fast inlet on a small domain): named, and their cells counted apart.
     diverged = [k for k in range(len(doms)) if not np.isfinite(b.reduce(k)).all()]
     finite_cells = sum(d.nx * d.ny for k, d in enumerate(doms) if k not in set(diverged))
+    acc = None
+    if args.accuracy_steps > 0:
+        acc = accuracy_batch(b, doms, [k for k in range(len(doms)) if k not in set(diverged)], args.accuracy_steps)
+        log(f"batch accuracy vs binary64 oracle: {acc}")
     log(f"batch: {K} steps {ms:.1f} ms, {cells / step_s / 1e6:.0f} MLUPS; {len(doms) - len(diverged)} of "
         f"{len(doms)} domains finite after {args.warmup + K} steps; diverged: {diverged}")
     line = {
@@ -239,6 +352,7 @@ def run_batch(args):
                    "finite_domains": len(doms) - len(diverged), "diverged_domains": diverged,
                    "finite_cells": int(finite_cells)},
         "roofline": roof,
+        "accuracy": acc,
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -283,6 +397,11 @@ def run_d3q19(args):
         f"mass {r[0]:.9e} drag ({r[4]:.3e}, {r[5]:.3e}, {r[6]:.3e})")
     if not np.isfinite(r).all():
         raise SystemExit("non-finite state after the timed run")
+    acc = None
+    if args.accuracy_steps > 0:
+        # the numpy oracle takes about 5 s per step at 256^3
+        acc = accuracy_d3q19(lat, scene, tau, (g, 0.0, 0.0), min(args.accuracy_steps, 5))
+        log(f"d3q19 accuracy vs binary64 oracle: {acc}")
     block, cpt, flags = L.variant3_shapes()[variant]
     line = {
         "metric": METRIC,
@@ -303,6 +422,7 @@ def run_d3q19(args):
                    "tile_variant": {"kernel": "k3_step", "block": block, "cells_per_thread": cpt, "flags": flags},
                    "graph_chunk": args.graph_chunk},
         "roofline": roof,
+        "accuracy": acc,
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -323,6 +443,10 @@ def run_learned(args):
                capacity=args.learned_capacity, train_steps=args.learned_train_steps, batch=args.learned_batch,
                log=log)
     nc, K = n // 4, r.coarse_steps
+    acc = None
+    if args.accuracy_steps > 0:
+        acc = accuracy_coarse(r, args.accuracy_steps, 2)
+        log(f"coarse accuracy vs binary64 oracles: {acc}")
     line = {
         "metric": METRIC,
         "value": round(nc * nc * K / r.learned_ms / 1e3, 1),
@@ -357,6 +481,7 @@ def run_learned(args):
                                              round(r.fine_ms / (r.learned_ms + 1e3 * (r.collect_s + r.train_s)), 3)},
         },
         "roofline": roofline(72, nc * nc, r.learned_ms / 1e3 / K, 1, None),
+        "accuracy": acc,
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

@@ -61,6 +61,14 @@ class LearnedResult:
     learned_f16_err: float = float("nan")
     coarse_steps: int = 0
     coarse_health: dict = field(default_factory=dict)
+    # the coarse problem and the trained network at the timed learned run's
+    # gain, for checks after the workflow
+    fc0: np.ndarray | None = None
+    solid_c: np.ndarray | None = None
+    force_c: tuple = (0.0, 0.0)
+    net_params: np.ndarray | None = None
+    net_scales: tuple = (1.0, 1.0)
+    net_gain: float = 1.0
 
     @property
     def gated_err(self) -> float:
@@ -170,6 +178,9 @@ def run(n: int = 8192, fine_steps: int = 4000, force: float = 1e-7, pair_every: 
     res.learned_ms, res.learned_err, h_learn = coarse(True, res.coarse_steps, (uxr, uyr))
     res.learned_f16_ms, res.learned_f16_err, _ = coarse(True, res.coarse_steps, (uxr, uyr), L.MLP_F16)
     res.coarse_health = {"plain": h_plain, "learned": h_learn}
+    res.fc0, res.solid_c, res.force_c = fc0, solid_c, gc
+    res.net_params, res.net_scales = net.params(), net.scales()
+    res.net_gain = res.gain if res.gain > 0.0 else 1.0
     net.close()
     log(f"coarse plain: {res.plain_ms:.1f} ms, vel L2 {res.plain_err:.4e}; learned (gain "
         f"{res.gain if res.gain > 0 else 1:g}, bf16 operands): {res.learned_ms:.1f} ms, vel L2 "
