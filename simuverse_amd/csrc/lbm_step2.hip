@@ -518,6 +518,7 @@ const BatchVariant kBatchVariants[] = {
     {512, 64, 0, kNtStore | kThreeStep, 16}, {512, 64, 0, kNtStore | kThreeStep | kXcdRemap, 16},
     {256, 32, 0, kNtStore | kThreeStep, 16}, {256, 32, 0, kNtStore | kThreeStep | kXcdRemap, 16},
     {256, 64, 0, kNtStore | kThreeStep, 8},
+    {768, 64, 0, kNtStore | kThreeStep, 24}, {768, 64, 0, kNtStore | kThreeStep | kXcdRemap, 24},
 };
 const int kNumBatchVariants = (int)(sizeof(kBatchVariants) / sizeof(kBatchVariants[0]));
 

@@ -205,12 +205,14 @@ static hipError_t dispatch_variant3(int v, const PhysArgs& a, const void* src, v
     case 43: return launch_phys_variant3<T, XEDGE, FORCE, 512, 128, 8, S>(a, src, dst, bits, words, s);
     case 44: return launch_phys_variant3<T, XEDGE, FORCE, 256, 32, 16, S>(a, src, dst, bits, words, s);
     case 45: return launch_phys_variant3<T, XEDGE, FORCE, 256, 64, 8, S>(a, src, dst, bits, words, s);
+    case 46: return launch_phys_variant3<T, XEDGE, FORCE, 768, 64, 24, S>(a, src, dst, bits, words, s);
+    case 47: return launch_phys_variant3<T, XEDGE, FORCE, 768, 64, 24, S | R>(a, src, dst, bits, words, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-// The region-1 LDS buffer of every shape stays within 64 KiB in binary32
-// (128 x 8: 57 024 B; 64 x 16: 48 960 B).
+// The region-1 LDS buffer in binary32: 64 x 16: 48 960 B (3 blocks of 512
+// threads per CU), 128 x 8: 57 024 B, 64 x 24: 68 544 B (2 blocks of 768).
 hipError_t launch_phys_step3(int variant, bool f16, int xedge, bool force, const PhysArgs& a, const void* src,
                              void* dst, const uint32_t* bits, uint32_t words, hipStream_t s)
 {
@@ -268,6 +270,8 @@ static hipError_t launch_batch3_typed(int v, const BatchDomainDev* doms, const u
     case 25: LBM_BATCH3(256, 32, 16, kNtStore); break;
     case 26: LBM_BATCH3(256, 32, 16, kNtStore | kXcdRemap); break;
     case 27: LBM_BATCH3(256, 64, 8, kNtStore); break;
+    case 28: LBM_BATCH3(768, 64, 24, kNtStore); break;
+    case 29: LBM_BATCH3(768, 64, 24, kNtStore | kXcdRemap); break;
     default: return hipErrorInvalidValue;
     }
 #undef LBM_BATCH3

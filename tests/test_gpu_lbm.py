@@ -91,9 +91,9 @@ def test_tiny_periodic_lattices_against_oracles(nx, ny):
     against the WGSL restatement (both buffers, every slot: no solids), and
     physics mode with a body force (and a solid cell when the lattice has
     room), with the default one-step shape, the 128 x 8 two-step tile, the
-    512-column two-step row stream and the 64 x 16 three-step tile (its
-    two-cell ring wraps around the lattice more than once here), against
-    the binary64 oracle; after 1, 2, 7 and 100 steps."""
+    512-column two-step row stream and the 64 x 16 and 64 x 24 three-step
+    tiles (their two-cell ring wraps around the lattice more than once
+    here), against the binary64 oracle; after 1, 2, 7 and 100 steps."""
     from test_oracle_wgsl import bulk_info
     print(f"case: tiny lattice {nx}x{ny}", flush=True)
     rng = np.random.default_rng(100 * nx + ny)
@@ -113,7 +113,7 @@ def test_tiny_periodic_lattices_against_oracles(nx, ny):
         st = O.PhysState(O.phys_params(nx, ny, 0.7, force=force), solid, f0)
         st.run(steps)
         r0, u0, v0 = st.macro()
-        for variant in (None, 18, 34, 41):
+        for variant in (None, 18, 34, 41, 47):
             lat = L.Lattice(nx, ny, 0.7, force=force)
             lat.write_materials(W.materials_from_solid(solid))
             lat.write_populations(f0.astype(np.float32), 0)
