@@ -519,6 +519,7 @@ const BatchVariant kBatchVariants[] = {
     {256, 32, 0, kNtStore | kThreeStep, 16}, {256, 32, 0, kNtStore | kThreeStep | kXcdRemap, 16},
     {256, 64, 0, kNtStore | kThreeStep, 8},
     {768, 64, 0, kNtStore | kThreeStep, 24}, {768, 64, 0, kNtStore | kThreeStep | kXcdRemap, 24},
+    {1024, 64, 0, kNtStore | kThreeStep, 32},
 };
 const int kNumBatchVariants = (int)(sizeof(kBatchVariants) / sizeof(kBatchVariants[0]));
 

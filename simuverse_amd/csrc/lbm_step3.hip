@@ -272,6 +272,7 @@ static hipError_t launch_batch3_typed(int v, const BatchDomainDev* doms, const u
     case 27: LBM_BATCH3(256, 64, 8, kNtStore); break;
     case 28: LBM_BATCH3(768, 64, 24, kNtStore); break;
     case 29: LBM_BATCH3(768, 64, 24, kNtStore | kXcdRemap); break;
+    case 30: LBM_BATCH3(1024, 64, 32, kNtStore); break;
     default: return hipErrorInvalidValue;
     }
 #undef LBM_BATCH3
