@@ -100,7 +100,7 @@ def rho_u_errors(pairs):
         sr += float(np.sum(dr ** 2)); sr0 += float(np.sum(ref[0][fl] ** 2))
         su += float(np.sum(du ** 2)); su0 += float(np.sum(uo ** 2))
         mr = max(mr, float(np.max(np.abs(dr)))); mu = max(mu, float(np.max(du))); umax = max(umax, float(np.max(uo)))
-    return {"rho_l2_rel": float(np.sqrt(sr / sr0)), "rho_max_abs": mr,
+    return {"rho_l2_rel": float(np.sqrt(sr / max(sr0, 1e-300))), "rho_max_abs": mr,
             "u_l2_rel": float(np.sqrt(su / max(su0, 1e-300))), "u_max_abs": mu, "u_max": umax}
 
 
@@ -142,6 +142,8 @@ def accuracy_batch(b, doms, finite, steps: int = 20, every: int = 8):
 
     t0 = time.perf_counter()
     sample = [k for k in finite[::every]]
+    if not sample:
+        return None
     states = {}
     for k in sample:
         d = doms[k]
