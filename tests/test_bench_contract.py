@@ -60,3 +60,31 @@ def test_roofline_fraction_is_per_launch():
         m = re.search(r"roofline frac ([0-9.]+)", p.read_text())
         if m:
             assert float(m.group(1)) <= 1.0, p.name
+
+
+def test_rho_u_errors_pool_domains():
+    """The accuracy legs pool several domains: the relative L2 errors are
+    sums of squares over every sampled fluid cell (not a mean of per-domain
+    ratios), solid cells are left out, and u compares the velocity vectors
+    (2-D or 3-D)."""
+    import numpy as np
+
+    bench = load_bench()
+    fl = np.array([[True, True], [True, False]])
+    ref1 = (np.full((2, 2), 1.0), np.full((2, 2), 0.1), np.zeros((2, 2)))
+    dev1 = (ref1[0] + 1e-3, ref1[1] + 3e-4, ref1[2] + 4e-4)  # |du| = 5e-4
+    dev1[0][1, 1] = 99.0  # a solid cell: ignored
+    ref2 = (np.full((1, 3), 2.0), np.full((1, 3), 0.2), np.zeros((1, 3)))
+    dev2 = (ref2[0], ref2[1], ref2[2])
+    out = bench.rho_u_errors([(dev1, ref1, fl), (dev2, ref2, np.ones((1, 3), bool))])
+    # rho: 3 cells off by 1e-3 against 3 cells of 1 and 3 of 2
+    assert abs(out["rho_l2_rel"] - np.sqrt(3e-6 / (3 + 12))) < 1e-12
+    assert abs(out["rho_max_abs"] - 1e-3) < 1e-12
+    # u: 3 cells off by 5e-4 against 3 of 0.1 and 3 of 0.2
+    assert abs(out["u_l2_rel"] - np.sqrt(3 * 25e-8 / (3 * 0.01 + 3 * 0.04))) < 1e-12
+    assert abs(out["u_max_abs"] - 5e-4) < 1e-12 and abs(out["u_max"] - 0.2) < 1e-12
+    # three components
+    r3 = (np.ones(4), np.zeros(4), np.zeros(4), np.full(4, 0.1))
+    d3 = (np.ones(4), np.full(4, 2e-4), np.zeros(4), np.full(4, 0.1))
+    o3 = bench.rho_u_errors([(d3, r3, np.ones(4, bool))])
+    assert o3["rho_l2_rel"] == 0.0 and abs(o3["u_max_abs"] - 2e-4) < 1e-15
