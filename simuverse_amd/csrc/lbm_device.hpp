@@ -220,6 +220,20 @@ struct PhysArgs {
     float sfa[9], sfb[9], sfc[3], sow[3];
 };
 
+// The Guo constants fa_i of the four direction pairs (1, 2, 5, 6; the
+// scaled set when SC) held in vector registers.  The collision's
+// D = ow_k d + fa_i has two wave-uniform operands, and a vector instruction
+// reads at most one scalar register, so each cell would first copy fa_i
+// into a vector register; pinned once per thread, the copies leave the
+// per-cell code.  A tile kernel calls this on its own copy of the
+// arguments.
+template <bool SC>
+__device__ inline void pin_pair_consts(PhysArgs& a)
+{
+    float* fa = SC ? a.sfa : a.fa;
+    asm volatile("" : "+v"(fa[1]), "+v"(fa[2]), "+v"(fa[5]), "+v"(fa[6]));
+}
+
 // Fills fcoef and the Guo coefficient tables from omega, gx, gy.
 __host__ __device__ inline void set_force_terms(PhysArgs& a)
 {

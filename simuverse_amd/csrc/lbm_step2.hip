@@ -30,10 +30,12 @@ template <int BW, int BH>
 constexpr int kRegionCells = (BW + 2) * (BH + 2);
 
 template <typename T, int XEDGE, bool FORCE, int BS, int BW, int BH, int FLAGS>
-__device__ inline void two_step_tile(const PhysArgs& a, const T* __restrict__ src, T* __restrict__ dst,
+__device__ inline void two_step_tile(const PhysArgs& a_in, const T* __restrict__ src, T* __restrict__ dst,
                                      const uint32_t* __restrict__ bits, uint32_t words, int x0, int y0,
                                      T (*__restrict__ lds)[kRegionCells<BW, BH>])
 {
+    PhysArgs a = a_in;
+    if (FORCE) pin_pair_consts<kScaledDom<T>>(a);
     constexpr int RW = BW + 2, RC = kRegionCells<BW, BH>;
     constexpr int kIter1 = (RC + BS - 1) / BS, kIter2 = (BW * BH) / BS;
     static_assert((BW * BH) % BS == 0, "tile must be a multiple of the block");

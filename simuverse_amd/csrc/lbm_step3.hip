@@ -28,10 +28,12 @@ template <int BW, int BH, int R>
 constexpr int kRegion = (BW + 2 * R) * (BH + 2 * R);
 
 template <typename T, int XEDGE, bool FORCE, int BS, int BW, int BH, int FLAGS>
-__device__ inline void three_step_tile(const PhysArgs& a, const T* __restrict__ src, T* __restrict__ dst,
+__device__ inline void three_step_tile(const PhysArgs& a_in, const T* __restrict__ src, T* __restrict__ dst,
                                        const uint32_t* __restrict__ bits, uint32_t words, int x0, int y0,
                                        T (*__restrict__ lds)[kRegion<BW, BH, 2>])
 {
+    PhysArgs a = a_in;
+    if (FORCE) pin_pair_consts<kScaledDom<T>>(a);
     constexpr int RW1 = BW + 4, RC1 = kRegion<BW, BH, 2>;
     constexpr int RW2 = BW + 2, RC2 = kRegion<BW, BH, 1>;
     constexpr int kIter1 = (RC1 + BS - 1) / BS, kIter2 = (RC2 + BS - 1) / BS, kIter3 = (BW * BH) / BS;
