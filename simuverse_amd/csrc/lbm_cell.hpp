@@ -83,6 +83,19 @@ __device__ __forceinline__ CellNbr cell_nbr(const PhysArgs& a, const uint32_t* _
     return nb;
 }
 
+// The same for a cell whose rows y - 1 .. y + 1 lie inside the lattice
+// (no wrap).
+__device__ __forceinline__ CellNbr cell_nbr_inner(const uint32_t* __restrict__ bits, uint32_t words, int x, int y)
+{
+    CellNbr nb;
+    nb.ym = y - 1;
+    nb.yp = y + 1;
+    nb.sm = solid3(bits, words, y - 1, x);
+    nb.s0 = solid3(bits, words, y, x);
+    nb.sp = solid3(bits, words, y + 1, x);
+    return nb;
+}
+
 // Zou & He velocity inlet on the gathered deviations of a column-0 cell
 // (w0+w2+w4+2(w3+w6+w7) = 1); SC: g is scaled, and the unscaled sums and
 // terms are formed exactly (power-of-two factors) so both forms agree.

@@ -131,11 +131,18 @@ __device__ inline void three_step_tile(const PhysArgs& a_in, const T* __restrict
         if (k >= RC2) continue;
         const int r = k / RW2, q = k % RW2;
         int x = x0 - 1 + q, y = y0 - 1 + r;
-        if (x > nx || y > ny) continue;                  // beyond the ring of a cut tile
-        if (XEDGE == 1 && (x < 0 || x >= nx)) continue;  // outside the channel: never read
-        x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
-        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
-        const CellNbr nb = cell_nbr(a, bits, words, x, y);
+        CellNbr nb;
+        if (inner) {
+            // region 2 and its neighbours lie inside the lattice, away from
+            // the inlet and outlet columns: no wrap, no cut
+            nb = cell_nbr_inner(bits, words, x, y);
+        } else {
+            if (x > nx || y > ny) continue;                  // beyond the ring of a cut tile
+            if (XEDGE == 1 && (x < 0 || x >= nx)) continue;  // outside the channel: never read
+            x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+            y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+            nb = cell_nbr(a, bits, words, x, y);
+        }
         if (nb.s0 & 2u) continue;  // solid: step 3 never reads its entries
         gather_lds(RW1, (r + 1) * RW1 + (q + 1), x, y, nb, g2[m]);
         float rho, ux, uy;
@@ -159,8 +166,13 @@ __device__ inline void three_step_tile(const PhysArgs& a_in, const T* __restrict
         const int k = (int)threadIdx.x + BS * m;
         const int row = k / BW, col = k % BW;
         const int x = x0 + col, y = y0 + row;
-        if (x >= nx || y >= ny) continue;
-        const CellNbr nb = cell_nbr(a, bits, words, x, y);
+        CellNbr nb;
+        if (inner) {
+            nb = cell_nbr_inner(bits, words, x, y);
+        } else {
+            if (x >= nx || y >= ny) continue;
+            nb = cell_nbr(a, bits, words, x, y);
+        }
         if (nb.s0 & 2u) continue;
         float g[9];
         gather_lds(RW2, (row + 1) * RW2 + (col + 1), x, y, nb, g);
