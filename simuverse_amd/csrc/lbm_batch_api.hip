@@ -154,7 +154,7 @@ bool batch_variant_ok(const lbm_batch* b, int v)
     if (v < 0 || v >= kNumBatchVariants) return false;
     if (batch_variant_steps(v) == 1) return true;
     for (const lbm_domain_desc& q : b->desc)
-        if (q.x_edge != LBM_X_PERIODIC && q.nx % kBatchVariants[v].vec == 1) return false;
+        if (!two_step_fits(kBatchVariants[v].vec, q.nx, q.ny, q.x_edge != LBM_X_PERIODIC ? 1 : 0)) return false;
     return true;
 }
 

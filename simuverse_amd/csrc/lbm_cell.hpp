@@ -84,15 +84,30 @@ __device__ __forceinline__ CellNbr cell_nbr(const PhysArgs& a, const uint32_t* _
 }
 
 // The same for a cell whose rows y - 1 .. y + 1 lie inside the lattice
-// (no wrap).
+// (no wrap).  MUL24, for lattices with y, words < 2^24 (the tile kernels'
+// fit rule): one 24-bit product (full rate; a 32-bit one issues at a
+// quarter) and the neighbour rows' windows one `words` either side.
+template <bool MUL24>
 __device__ __forceinline__ CellNbr cell_nbr_inner(const uint32_t* __restrict__ bits, uint32_t words, int x, int y)
 {
     CellNbr nb;
     nb.ym = y - 1;
     nb.yp = y + 1;
-    nb.sm = solid3(bits, words, y - 1, x);
-    nb.s0 = solid3(bits, words, y, x);
-    nb.sp = solid3(bits, words, y + 1, x);
+    if constexpr (MUL24) {
+        const uint32_t* p = bits + (__umul24((uint32_t)y, words) + ((uint32_t)x >> 5));
+        const uint32_t sh = (uint32_t)x & 31u;
+        auto win = [sh](const uint32_t* q) {
+            const uint64_t v = (uint64_t)q[0] | ((uint64_t)q[1] << 32);
+            return (uint32_t)(v >> sh) & 7u;
+        };
+        nb.sm = win(p - words);
+        nb.s0 = win(p);
+        nb.sp = win(p + words);
+    } else {
+        nb.sm = solid3(bits, words, y - 1, x);
+        nb.s0 = solid3(bits, words, y, x);
+        nb.sp = solid3(bits, words, y + 1, x);
+    }
     return nb;
 }
 
@@ -175,6 +190,19 @@ __device__ __forceinline__ void cell_push(const PhysArgs& a, T* __restrict__ dst
         const uint32_t trow = ey == 1 ? rp : (ey == -1 ? rm : r0);
         Dom<T, SC>::push(dst, trow + (uint32_t)tx + (uint32_t)inv_of(i) * a.plane, g[i], w_phys(i), a.scale);
     }
+}
+
+// cell_store with the cell's own element index y * rs + x given.
+template <typename T, int XEDGE, int FLAGS, bool SC = false>
+__device__ __forceinline__ void cell_store_at(const PhysArgs& a, T* __restrict__ dst, const CellNbr& nb, int x, int y,
+                                              uint32_t own_index, const float g[9])
+{
+    // one per-lane pointer; the direction offsets are uniform across the wave
+    T* const own = dst + own_index;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+        st<(FLAGS & kNtStore) != 0>(own + (size_t)((uint32_t)i * a.plane), Dom<T, SC>::out(g[i], a.scale));
+    cell_push<T, XEDGE, SC>(a, dst, nb, x, y, g);
 }
 
 template <typename T, int XEDGE, int FLAGS, bool SC = false>

@@ -421,13 +421,16 @@ __global__ __launch_bounds__(BS) void k_batch_step2(const BatchDomainDev* __rest
 // Launchers
 // ---------------------------------------------------------------------------
 
-// Whether the two-step tiles of width bw fit an nx x ny lattice: the ring
-// wraps in y (and in x when periodic), so nx, ny >= 3, and with the
-// inlet/outlet edge the last tile of a row holds at least two columns (the
-// outlet gathers through its west neighbour's ring).
+// Whether the two- and three-step tiles of width bw fit an nx x ny
+// lattice: the ring wraps in y (and in x when periodic), so nx, ny >= 3,
+// and with the inlet/outlet edge the last tile of a row holds at least two
+// columns (the outlet gathers through its west neighbour's ring).  The
+// three-step tile forms row offsets with 24-bit products, so rows, the row
+// stride (at most 9 (nx + 16) elements) and the mask words stay below 2^24.
 bool two_step_fits(int bw, int nx, int ny, int xedge)
 {
-    return nx >= 3 && ny >= 3 && (xedge == 0 || nx % bw != 1);
+    return nx >= 3 && ny >= 3 && (xedge == 0 || nx % bw != 1) && 9ll * (nx + 16) < (1ll << 24) &&
+           ny < (1 << 24);
 }
 
 template <typename T, int XEDGE, bool FORCE, int BS, int BW, int BH, int FLAGS>
