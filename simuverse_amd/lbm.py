@@ -248,13 +248,15 @@ def variant_shapes() -> list[tuple[int, int, int, int]]:
 
 def variant_ok(variant: int, nx: int, ny: int, x_edge: int = X_PERIODIC) -> bool:
     """Whether a tile variant can step an nx x ny lattice (the library's
-    own rule: two- and three-step variants need nx, ny >= 3 and, with the
-    inlet/outlet edge, a last tile of at least two columns (nx % vec != 1); lane-shuffle
-    variants need periodic x and whole blocks of block*vec cells per row;
-    vector variants need nx % vec == 0)."""
+    own rule: two- and three-step variants need nx, ny >= 3, with the
+    inlet/outlet edge a last tile of at least two columns (nx % vec != 1),
+    and rows and row stride below 2^24 (ny < 2^24, 9 (nx + 16) < 2^24);
+    lane-shuffle variants need periodic x and whole blocks of block*vec
+    cells per row; vector variants need nx % vec == 0)."""
     b, c, v, f = variant_shapes()[variant]
     if f & (16 | 64):
-        return nx >= 3 and ny >= 3 and (x_edge == X_PERIODIC or nx % v != 1)
+        return (nx >= 3 and ny >= 3 and (x_edge == X_PERIODIC or nx % v != 1) and 9 * (nx + 16) < (1 << 24)
+                and ny < (1 << 24))
     if f & 8:
         return x_edge == X_PERIODIC and nx % (b * v) == 0
     return v == 1 or nx % v == 0
