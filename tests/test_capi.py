@@ -100,3 +100,25 @@ def test_host_layer_library_loads():
     path = ROOT / "simuverse_amd" / "_lib" / "libsimuverse_host.so"
     assert path.exists(), "libsimuverse_host.so not built"
     C.CDLL(str(path))
+
+
+def test_tile_fit_rule():
+    """The Python mirror of the library's rule for multi-step tile shapes
+    (lbm_set_variant refuses a shape the rule rejects): nx, ny >= 3, with
+    the inlet/outlet edge a last tile of at least two columns, and rows and
+    row stride below 2^24 (the three-step tile's 24-bit row products)."""
+    from simuverse_amd import lbm as L
+
+    shapes = L.variant_shapes()
+    three = [v for v, s in enumerate(shapes) if s[3] & 64]
+    assert three
+    for v in three:
+        bw = shapes[v][2]
+        assert L.variant_ok(v, 4096, 4096)
+        assert L.variant_ok(v, 3, 3)
+        assert not L.variant_ok(v, 2, 64) and not L.variant_ok(v, 64, 2)
+        assert not L.variant_ok(v, bw + 1, 64, L.X_INLET_OUTLET)
+        assert L.variant_ok(v, bw + 1, 64, L.X_PERIODIC)
+        assert L.variant_ok(v, 3, (1 << 24) - 1) and not L.variant_ok(v, 3, 1 << 24)
+        big = (1 << 24) // 9 - 16
+        assert L.variant_ok(v, big - 1, 3) and not L.variant_ok(v, big + 1, 3)
