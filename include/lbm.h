@@ -255,7 +255,9 @@ int lbm_variant_info(int variant, int* block, int* cells_per_thread, int* vec, i
 int lbm_set_variant(lbm_handle* h, int variant);
 int lbm_get_variant(lbm_handle* h, int* variant);
 /* Times every tile shape over `steps` steps on this handle's lattice and
- * keeps the fastest.  The state is restored afterwards. */
+ * keeps the fastest; a shape later in the table replaces the best so far
+ * only when it is at least 1 % faster, so shapes of equal speed do not
+ * swap places between runs.  The state is restored afterwards. */
 int lbm_autotune(lbm_handle* h, int steps, int* best_variant, double* ms_per_step);
 /* Steps per HIP-graph chunk used by lbm_run (even, default 100; 0 launches
  * every step directly). */

@@ -860,7 +860,7 @@ int lbm3d_autotune(lbm3d_handle* h, int steps, int* best_variant, double* ms_per
         if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
         float t = 0.0f;
         if (e == hipSuccess) e = hipEventElapsedTime(&t, h->ev0, h->ev1);
-        if (e == hipSuccess && t / steps < best_ms) {
+        if (e == hipSuccess && t / steps < best_ms * kTuneMargin) {
             best_ms = t / steps;
             best = v;
         }

@@ -829,7 +829,7 @@ int lbm_autotune(lbm_handle* h, int steps, int* best_variant, double* ms_per_ste
         if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
         float t = 0.0f;
         if (e == hipSuccess) e = hipEventElapsedTime(&t, h->ev0, h->ev1);
-        if (e == hipSuccess && t / (launches * per) < best_ms) {
+        if (e == hipSuccess && t / (launches * per) < best_ms * kTuneMargin) {
             best_ms = t / (launches * per);
             best = v;
         }

@@ -592,7 +592,7 @@ int lbm_batch_autotune(lbm_batch* b, int steps, int* best_variant, double* ms_pe
         if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
         float t = 0.0f;
         if (e == hipSuccess) e = hipEventElapsedTime(&t, b->ev0, b->ev1);
-        if (e == hipSuccess && t / (launches * per) < best_ms) {
+        if (e == hipSuccess && t / (launches * per) < best_ms * kTuneMargin) {
             best_ms = t / (launches * per);
             best = v;
         }

@@ -23,6 +23,12 @@ struct StepVariant {
 // (launch_phys_step2) or three (launch_phys_step3), and the steps left over
 // use the one-step variant kSingleStepFallback.
 constexpr int kSingleStepFallback = 6;
+// The autotuners keep the fastest shape, but a later shape in the table
+// replaces the best so far only when it is at least 1 % faster: shapes of
+// nearly equal speed would otherwise swap places between runs on timing
+// noise.  The tables list the preferred of such twins first (the XCD-
+// remapped 64 x 24 tile, which reads one pass, before the unmapped one).
+constexpr double kTuneMargin = 0.99;
 extern const StepVariant kStepVariants[];
 extern const int kNumStepVariants;
 bool step_variant_ok(int variant, int nx, int ny, int xedge);

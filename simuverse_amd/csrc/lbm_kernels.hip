@@ -791,8 +791,8 @@ const StepVariant kStepVariants[] = {
     {512, 8, 128, kNtStore | kThreeStep},
     {256, 16, 32, kNtStore | kThreeStep},
     {256, 8, 64, kNtStore | kThreeStep},
-    {768, 24, 64, kNtStore | kThreeStep},
     {768, 24, 64, kNtStore | kThreeStep | kXcdRemap},
+    {768, 24, 64, kNtStore | kThreeStep},
 };
 const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
 

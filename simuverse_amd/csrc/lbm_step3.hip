@@ -221,8 +221,8 @@ static hipError_t dispatch_variant3(int v, const PhysArgs& a, const void* src, v
     case 43: return launch_phys_variant3<T, XEDGE, FORCE, 512, 128, 8, S>(a, src, dst, bits, words, s);
     case 44: return launch_phys_variant3<T, XEDGE, FORCE, 256, 32, 16, S>(a, src, dst, bits, words, s);
     case 45: return launch_phys_variant3<T, XEDGE, FORCE, 256, 64, 8, S>(a, src, dst, bits, words, s);
-    case 46: return launch_phys_variant3<T, XEDGE, FORCE, 768, 64, 24, S>(a, src, dst, bits, words, s);
-    case 47: return launch_phys_variant3<T, XEDGE, FORCE, 768, 64, 24, S | R>(a, src, dst, bits, words, s);
+    case 46: return launch_phys_variant3<T, XEDGE, FORCE, 768, 64, 24, S | R>(a, src, dst, bits, words, s);
+    case 47: return launch_phys_variant3<T, XEDGE, FORCE, 768, 64, 24, S>(a, src, dst, bits, words, s);
     default: return hipErrorInvalidValue;
     }
 }
