@@ -454,14 +454,16 @@ def test_f16_deviation_storage_vs_fp64():
     assert e["rho_l2"] < 1e-5 and e["u_l2"] < 5e-2
 
 
-@pytest.mark.parametrize("n, storage, variant", [(4096, L.STORAGE_F32, None), (4096, L.STORAGE_F32, 24),
-                                               (8192, L.STORAGE_F16_DEV, 26)],
-                         ids=["4096-f32-default", "4096-f32-two-step", "8192-f16-two-step"])
+@pytest.mark.parametrize("n, storage, variant", [(4096, L.STORAGE_F32, None), (4096, L.STORAGE_F32, 46),
+                                               (8192, L.STORAGE_F16_DEV, 38), (8192, L.STORAGE_F16_DEV, 42)],
+                         ids=["4096-f32-default", "4096-f32-three-step", "8192-f16-row-stream",
+                              "8192-f16-three-step"])
 def test_full_size_mass_and_momentum_properties(n, storage, variant):
     """The roofline lattices at full size (periodic, disks >= 5 %,
     g = 1e-6, tau = 0.55; 4096^2 binary32 with the default one-step shape and
-    the two-step shape the benchmark tunes to, 8192^2 binary16 deviations
-    with its two-step shape): the binary64 reduction shows mass conserved
+    the three-step 64 x 24 tile the benchmark tunes to, 8192^2 binary16
+    deviations with the two shapes its tuning settles on, the row stream and
+    the three-step 64 x 16 tile): the binary64 reduction shows mass conserved
     over 100 steps and the momentum identity P(t+1) - P(t) = g N - drag for
     one more step.  The identity holds to the rounding of one step, bounded
     from the state itself (momentum_rounding_bound): every stored deviation
@@ -491,6 +493,34 @@ def test_full_size_mass_and_momentum_properties(n, storage, variant):
           f"of g N, {residual / bound:.2e} of its rounding bound ({bound / (g * nfl):.2e} of g N)", flush=True)
     assert residual <= bound
     assert bound < 0.1 * g * nfl  # the identity is still a sharp check of force and drag
+
+
+@pytest.mark.parametrize("n, storage, variant", [(4096, L.STORAGE_F32, 46), (8192, L.STORAGE_F16_DEV, 38),
+                                               (8192, L.STORAGE_F16_DEV, 42)],
+                         ids=["4096-f32-three-step", "8192-f16-row-stream", "8192-f16-three-step"])
+def test_full_size_multi_step_shapes_match_one_step(n, storage, variant):
+    """At the benchmark's own sizes the shapes it tunes to give the one-step
+    kernel's populations bit for bit: 30 steps from the same seeded disk
+    lattice (10 three-step or 15 two-step launches against 30 one-step
+    launches) and one step more, compared at every slot a fluid cell can
+    read."""
+    print(f"case: {n}x{n} shape {variant} against one step, storage {storage}", flush=True)
+    scene = W.periodic_disks(n, n)
+    one_step = next(v for v, s in enumerate(L.variant_shapes()) if not s[3] & (16 | 64))
+    outs = []
+    for v in (one_step, variant):
+        lat = L.Lattice(n, n, 0.55, force=(1e-6, 0.0), storage=storage)
+        lat.write_materials(W.materials_from_solid(scene.solid))
+        lat.reset()
+        lat.set_variant(v)
+        lat.run(30)
+        lat.run(1)
+        outs.append(lat.read_populations())
+        lat.close()
+    vis = visible_mask(scene.solid, periodic_x=True)
+    assert np.isfinite(outs[0][vis]).all()
+    bad = np.count_nonzero(outs[1][vis] != outs[0][vis])
+    assert bad == 0, f"shape {variant} differs from the one-step kernel at {bad} of {int(vis.sum())} slots"
 
 
 def momentum_rounding_bound(f, solid, storage, shift=10):
