@@ -70,6 +70,17 @@ def traffic_from_profiles(workload: str, variant: int):
     return float(m.group(1))
 
 
+def traffic_from_profile_key(name: str, key: str):
+    """HBM bytes per launch recorded as `key: <bytes>` in profiles/<name>.md
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None."""
+    p = ROOT / "profiles" / f"{name}.md"
+    if not p.exists():
+        return None
+    m = re.search(rf"^{re.escape(key)}:\s*([0-9.eE+]+)", p.read_text(), re.M)
+    return float(m.group(1)) if m else None
+
+
 def roofline(bytes_per_cell: int, cells: int, step_s: float, per_launch: int, traffic):
     """The step kernel against the HBM roofline.  A launch that performs
     per_launch steps must still read and write every cell's populations
@@ -482,7 +493,9 @@ def run_learned(args):
                                          "learned_incl_training_vs_fine":
                                              round(r.fine_ms / (r.learned_ms + 1e3 * (r.collect_s + r.train_s)), 3)},
         },
-        "roofline": roofline(72, nc * nc, r.learned_ms / 1e3 / K, 1, None),
+        "roofline": roofline(72, nc * nc, r.learned_ms / 1e3 / K, 1,
+                             traffic_from_profile_key("d2q9_learned_mfma_2048x2048_trace",
+                                                      "hbm_bytes_per_launch_bf16") if nc == 2048 else None),
         "accuracy": acc,
         "cpu_baseline": None,
     }

@@ -41,6 +41,21 @@ def test_profile_summaries_match_their_names():
         assert traffic is not None and 1e8 < traffic < 1e11, (p.name, traffic)
 
 
+def test_learned_step_traffic_is_recorded():
+    """The learned workload's line reads the matrix-core step's measured
+    bytes per step from its profile; they sit between one pass of the
+    lattice (72 B per cell of the 2048^2 coarse lattice) and twice that."""
+    bench = load_bench()
+    name = "d2q9_learned_mfma_2048x2048_trace"
+    text = (ROOT / "profiles" / f"{name}.md").read_text()
+    assert "rocprofv3 --pmc FETCH_SIZE" in text and "rocprofv3 --pmc WRITE_SIZE" in text
+    one_pass = 72 * 2048 * 2048
+    for key in ("hbm_bytes_per_launch_bf16", "hbm_bytes_per_launch_f16"):
+        traffic = bench.traffic_from_profile_key(name, key)
+        assert traffic is not None and one_pass <= traffic < 2 * one_pass, (key, traffic)
+    assert bench.traffic_from_profile_key(name, "no_such_key") is None
+
+
 def test_roofline_fraction_is_per_launch():
     """A two-step launch must still read and write every population once:
     frac counts those bytes over the launch duration and stays <= 1 at the
