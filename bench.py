@@ -530,7 +530,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the lattice would shard by "
+        print("bench.py measures one GPU: the library has no multi-GPU path (the lattice would shard by "
               "y-slabs with a one-row halo exchange per step)", file=sys.stderr)
         sys.exit(2)
 

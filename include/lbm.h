@@ -51,7 +51,7 @@ enum {
     LBM_ERR_INVALID = 1,     /* bad argument or handle state            */
     LBM_ERR_HIP = 2,         /* a HIP runtime call failed               */
     LBM_ERR_OOM = 3,         /* device allocation failed                */
-    LBM_ERR_UNSUPPORTED = 4, /* valid request this build does not do    */
+    LBM_ERR_UNSUPPORTED = 4, /* valid request the library does not do  */
     LBM_ERR_NO_DEVICE = 5    /* no gfx950 device visible                */
 };
 

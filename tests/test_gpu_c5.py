@@ -43,7 +43,7 @@ def test_c5_workflow_1024_to_256():
     assert r.gain in LC.GAINS and r.gate_errors[r.gain] == min(r.gate_errors.values())
     for name, (umax, finite) in r.coarse_health.items():
         assert finite and umax < 0.3, name
-    # plain BGK is the judge: the gate's choice costs at most a little
+    # plain BGK is the yardstick: the gate's choice costs at most a little
     # against it past the window it saw, and both are within a few percent
     # of the restricted fine solution on this resolved flow
     assert r.gated_err <= 1.25 * r.plain_err
