@@ -314,6 +314,9 @@ int lbm_batch_variant_info(int variant, int* block, int* vec, int* tile_h, int* 
 int lbm_batch_set_variant(lbm_batch* b, int variant);
 int lbm_batch_get_variant(lbm_batch* b, int* variant);
 int lbm_batch_set_graph_chunk(lbm_batch* b, int chunk);
+/* Times every batched shape over `steps` steps on this batch and keeps the
+ * fastest, by the same 1 % rule as lbm_autotune; the state is restored
+ * afterwards. */
 int lbm_batch_autotune(lbm_batch* b, int steps, int* best_variant, double* ms_per_step);
 
 /* ---- learned collision correction --------------------------------------

@@ -81,8 +81,9 @@ int lbm3d_variant_count(void);
 int lbm3d_variant_info(int variant, int* block, int* cells_per_thread, int* flags);
 int lbm3d_set_variant(lbm3d_handle* h, int variant);
 int lbm3d_get_variant(lbm3d_handle* h, int* variant);
-/* Times every shape over `steps` steps and keeps the fastest; the state is
- * restored afterwards. */
+/* Times every shape over `steps` steps and keeps the fastest (a shape later
+ * in the table must be at least 1 % faster than the best so far); the state
+ * is restored afterwards. */
 int lbm3d_autotune(lbm3d_handle* h, int steps, int* best_variant, double* ms_per_step);
 /* Steps per HIP-graph chunk of lbm3d_run (even, default 100; 0: direct
  * launches). */
