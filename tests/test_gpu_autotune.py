@@ -1,5 +1,5 @@
-"""The tile-shape autotuners (lbm_autotune, lbm3d_autotune) through the C
-ABI: the shape they keep fits the lattice and becomes the handle's shape,
+"""The tile-shape autotuners (lbm_autotune, lbm_batch_autotune,
+lbm3d_autotune) through the C ABI: the shape they keep fits the lattice and becomes the handle's shape,
 the populations the caller had are restored bit for bit, and stepping on
 afterwards gives what a handle set to that shape by hand gives."""
 import numpy as np
@@ -89,3 +89,36 @@ def test_d3q19_autotune_keeps_a_fitting_shape_and_restores_the_state():
     bad = np.count_nonzero(tuned[vis] != ref.read_populations()[vis])
     ref.close()
     assert bad == 0, f"stepping after autotune differs from shape {v} set by hand at {bad} slots"
+
+
+@pytest.mark.parametrize("storage", [L.STORAGE_F32, L.STORAGE_F16_DEV], ids=["f32", "f16"])
+def test_batch_autotune_keeps_a_shape_and_restores_the_state(storage):
+    print(f"case: batch autotune, storage {storage}", flush=True)
+    doms = W.batch_domains(count=6, seed=11, size_min=64, size_max=256)
+    spec = [dict(nx=d.nx, ny=d.ny, tau=d.tau, x_edge=L.X_INLET_OUTLET, inlet_ux=d.inlet_ux) for d in doms]
+
+    def fresh():
+        b = L.Batch(spec, storage=storage)
+        for k, d in enumerate(doms):
+            b.write_materials(k, W.materials_from_solid(d.solid))
+        b.run(5)
+        return b
+
+    b = fresh()
+    before = [b.read_populations(k) for k in range(len(doms))]
+    v, ms = b.autotune(steps=6)
+    assert ms > 0.0 and b.variant() == v
+    for k in range(len(doms)):
+        assert np.array_equal(b.read_populations(k), before[k]), f"autotune changed domain {k}"
+    b.run(31)
+    tuned = [b.read_populations(k) for k in range(len(doms))]
+    b.close()
+
+    ref = fresh()
+    ref.set_variant(v)
+    ref.run(31)
+    for k, d in enumerate(doms):
+        vis = visible_mask(d.solid, periodic_x=False)
+        bad = np.count_nonzero(tuned[k][vis] != ref.read_populations(k)[vis])
+        assert bad == 0, f"domain {k}: stepping after autotune differs from shape {v} set by hand at {bad} slots"
+    ref.close()
